@@ -1,0 +1,103 @@
+"""ctypes binding of ``libgmaster_amd.so`` (the C-ABI declared in ``include/gmaster_amd.h``).
+
+The library is built in-tree by ``gmaster_amd.build``.  There is no fallback: if the
+library is missing every transform raises, naming the build command.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgmaster_amd.so")
+
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_p = ctypes.c_void_p
+
+# name -> argument types (all return int status unless listed in _RET_I64)
+SIGNATURES = {
+    "gm_ring_fft_fwd_s0": [_p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p],
+    "gm_ring_fft_fwd_s": [_p, _p, _i64, _i64, _p, _i64, _p],
+    "gm_ring_fft_inv_s0": [_p, _i64, _i64, _i64, _i64, _i64, _p, _i64, _p],
+    "gm_ring_fft_inv_s": [_p, _i64, _i64, _i64, _p, _p, _p],
+    "gm_ring_fold_residual": [_p, _p, _p, _i64, _i64, _i64, _i64, _i64, _p],
+    "gm_ring_fold_residual_c": [_p, _p, _p, _i64, _i64, _i64, _p],
+    "gm_lat_tables": [_i64, _i64, _p, _p, _p, _p],
+    "gm_lat_ana_s0": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
+    "gm_lat_finish_s0": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
+    "gm_lat_prep_s0": [_p, _i64, _i64, _p, _i64, _p, _p],
+    "gm_lat_syn_s0": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "gm_lat_ana_s": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
+    "gm_lat_finish_s": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _p],
+    "gm_lat_prep_s": [_p, _p, _p, _i64, _i64, _p, _p],
+    "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
+    "gm_coupled_cell": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p],
+    "gm_coupling": [_p, _p, _i64, _p, _p, _p, _p, _p],
+    "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
+    "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
+    "gm_table_size": [_i64],
+    "gm_lat_nchunks": [_i64, _i64],
+    "gm_cl_work_size": [_i64, _i64],
+    "gm_abi_version": [],
+    "gm_last_error": [],
+}
+_RET_I64 = {"gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
+
+_lib = None
+
+
+class GmError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+    def __init__(self, name, code, msg):
+        super().__init__(f"{name} failed (code {code}): {msg}")
+        self.code = code
+
+
+def load():
+    """Load the library (once) and declare every exported symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"gmaster_amd native library not found at {LIB_PATH}; build it with "
+            "`python -m gmaster_amd.build` (hipcc, gfx950)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        if name == "gm_last_error":
+            fn.restype = ctypes.c_char_p
+        elif name in _RET_I64:
+            fn.restype = _i64
+        else:
+            fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    lib = load()
+    code = getattr(lib, name)(*args)
+    if code != 0:
+        raise GmError(name, code, lib.gm_last_error().decode())
+    return code
+
+
+def query(name, *args):
+    return getattr(load(), name)(*args)
+
+
+def ptr(t):
+    """Device pointer of a tensor (or None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

@@ -1,0 +1,768 @@
+// Latitudinal (Legendre / Wigner-d) stage of the HEALPix transforms on gfx950, FP64.
+//
+// For each order m the normalised rows lambda_lm(theta) = sqrt((2l+1)/4pi) d^l_{m,-s}
+// are generated on the fly by the three-term recurrence in l, rescaled so that every
+// step is two FP64 VALU operations:
+//     mu_l = (alpha_l x + gamma_l) mu_{l-1} - mu_{l-2},   lambda_l = P_l mu_l
+// (alpha, gamma, P per (l, m) come from gm_lat_tables; gamma = 0 for spin 0).  Lanes are
+// rings: every lane carries R rings in registers (recurrence state + ring Fourier
+// coefficients), a workgroup of NW waves covers NW*64*R rings of one order m, and the
+// contraction over rings is reduced through LDS once per block of 4 degrees.
+//
+// Extended exponent: a ring whose seed lambda_M lies below 2^-900 starts with k < 0 and
+// mu = lambda 2^(600|k|) <= 2^-300; whenever |mu| passes 2^-300 the lane is rescaled by
+// 2^-600 and k moves towards 0.  Lanes with k < 0 therefore hold |mu| < 2^-268 and add
+// nothing measurable to any sum, so the contraction needs no per-lane select.
+//
+// Rings are ordered from the equator to the poles and the orders m whose polar-skip
+// bound mlim(theta) (ducc0's rule, docs/notes/latitudinal_march_maths.md:245-250) is
+// exceeded are simply not visited: nactive[m] is a prefix length.
+#include "common.h"
+
+namespace gm {
+
+namespace {
+
+constexpr double kInactiveMax = 0x1p-300;
+constexpr double kRescale = 0x1p-600;
+
+// lambda_M for order m at a ring as (mu, k): mant * 2^e from the host constant and
+// the powers sin^(m+s)(theta/2) cos^|m-s|(theta/2).
+__device__ inline void seed_lane(double smant, int sexp, int a, int b, double shalf, double chalf,
+                                 double& mu, int& k) {
+    double m1, m2;
+    int e1, e2;
+    pow_ext(shalf, a, m1, e1);
+    pow_ext(chalf, b, m2, e2);
+    const double v = smant * m1 * m2;
+    const int E = sexp + e1 + e2;
+    if (E >= -900) {
+        k = 0;
+        mu = ldexp(v, E);
+    } else {
+        const int kk = (-900 - E + 599) / 600;
+        k = -kk;
+        mu = ldexp(v, E + 600 * kk);
+    }
+}
+
+// ------------------------------------------------------------------ block reduction
+// acc[NV] per thread -> out[0..NV) = sum over all NW*64 threads; one barrier per call
+// (double-buffered LDS: `buf` alternates between calls).  Threads o*TPO .. (o+1)*TPO-1
+// sum output o, TPO = 64*NW / NV, each over NV consecutive thread slots.
+template <int NV, int NW>
+__device__ inline void block_reduce(const double (&acc)[NV], double* lds, int buf, double* out) {
+    constexpr int T = 64 * NW;
+    constexpr int TPO = T / NV;
+    double* base = lds + buf * (NV * T);
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) base[v * T + tid] = acc[v];
+    __syncthreads();
+    const int o = tid / TPO, sl = tid % TPO;
+    const double* src = base + o * T + sl * NV;
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < NV; ++u) s += src[u];
+#pragma unroll
+    for (int msk = 1; msk < TPO; msk <<= 1) s += __shfl_xor(s, msk);
+    if (sl == 0) out[o] = s;
+}
+
+template <int R>
+__device__ inline void rescale(double (&mu0)[R], double (&mu1)[R], int (&kx)[R], bool& anyinact) {
+    if (!anyinact) return;
+    bool still = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        if (kx[t] < 0 && fabs(mu0[t]) > kInactiveMax) {
+            mu0[t] *= kRescale;
+            mu1[t] *= kRescale;
+            kx[t] += 1;
+        }
+        still = still || (kx[t] < 0);
+    }
+    anyinact = __any(still);
+}
+
+// ------------------------------------------------------------------ spin-0 analysis
+// Folded lanes: j = 0 is the equator ring, j > 0 the pair (north r = 2n-1-j, south 2n-1+j).
+template <int K, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ spec, int64_t ld,
+                                                    int64_t smap, int nside, int L,
+                                                    const double* __restrict__ alpha,
+                                                    const double* __restrict__ smant,
+                                                    const int* __restrict__ sexp,
+                                                    const int* __restrict__ nact,
+                                                    double* __restrict__ partial, int64_t tsize) {
+    constexpr int NV = 4 * 2 * K;  // 4 degrees x (re, im) x maps
+    constexpr int T = 64 * NW;
+    constexpr int CH = T * R;
+    __shared__ double lds[2 * NV * T];
+    const int c = blockIdx.x, m = blockIdx.y;
+    const int na = nact[m];
+    if (c * CH >= na) return;
+
+    double x[R], mu0[R], mu1[R];
+    int kx[R];
+    double2 gE[R][K], gO[R][K];
+    const double sm = smant[m];
+    const int se = sexp[m];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        mu1[t] = 0.0;
+        if (j < na) {
+            const int r = 2 * nside - 1 - j, rp = 2 * nside - 1 + j;
+            const Ring g = ring_info(nside, r);
+            x[t] = g.z;
+            seed_lane(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const double2 a = spec[k * smap + (int64_t)r * ld + m];
+                if (j == 0) {
+                    gE[t][k] = a;
+                    gO[t][k] = make_double2(0.0, 0.0);
+                } else {
+                    const double2 b = spec[k * smap + (int64_t)rp * ld + m];
+                    gE[t][k] = cadd(a, b);
+                    gO[t][k] = csub(a, b);
+                }
+            }
+        } else {
+            x[t] = 0.0;
+            mu0[t] = 0.0;
+            kx[t] = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) gE[t][k] = gO[t][k] = make_double2(0.0, 0.0);
+        }
+    }
+    bool anyinact = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) anyinact = anyinact || (kx[t] < 0);
+    anyinact = __any(anyinact);
+
+    const double* al = alpha + table_base(m, L) - m;
+    double* part = partial + ((int64_t)c * tsize + table_base(m, L) - m) * (2 * K);
+    int buf = 0;
+    for (int l0 = m; l0 < L; l0 += 4, buf ^= 1) {
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            // l0 + m is even, so degree l0 + i pairs with the even (E) combination for even i
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const double2 gv = (i & 1) ? gO[t][k] : gE[t][k];
+                    acc[i * 2 * K + 2 * k] = fma(mu0[t], gv.x, acc[i * 2 * K + 2 * k]);
+                    acc[i * 2 * K + 2 * k + 1] = fma(mu0[t], gv.y, acc[i * 2 * K + 2 * k + 1]);
+                }
+            }
+            const double a = al[l0 + i + 1];
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const double nu = fma(x[t] * a, mu0[t], -mu1[t]);
+                mu1[t] = mu0[t];
+                mu0[t] = nu;
+            }
+        }
+        rescale<R>(mu0, mu1, kx, anyinact);
+        block_reduce<NV, NW>(acc, lds, buf, part + (int64_t)l0 * (2 * K));
+    }
+}
+
+// ------------------------------------------------------------------ spin-0 synthesis
+template <int K, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ coef, int nside, int L,
+                                                    const double* __restrict__ alpha,
+                                                    const double* __restrict__ smant,
+                                                    const int* __restrict__ sexp,
+                                                    const int* __restrict__ nact,
+                                                    double2* __restrict__ spec, int64_t ld, int64_t smap) {
+    constexpr int T = 64 * NW;
+    constexpr int CH = T * R;
+    const int c = blockIdx.x, m = blockIdx.y;
+    const int na = nact[m];
+    const int NF = 2 * nside;
+
+    double x[R], mu0[R], mu1[R];
+    int kx[R];
+    double2 aE[R][K], aO[R][K];
+    const double sm = smant[m];
+    const int se = sexp[m];
+    bool anyact = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        mu1[t] = 0.0;
+        kx[t] = 0;
+        x[t] = 0.0;
+        mu0[t] = 0.0;
+        if (j < na) {
+            const Ring g = ring_info(nside, 2 * nside - 1 - j);
+            x[t] = g.z;
+            seed_lane(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
+            anyact = true;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) aE[t][k] = aO[t][k] = make_double2(0.0, 0.0);
+    }
+    if (__any(anyact)) {
+        bool anyinact = false;
+#pragma unroll
+        for (int t = 0; t < R; ++t) anyinact = anyinact || (kx[t] < 0);
+        anyinact = __any(anyinact);
+        const double* al = alpha + table_base(m, L) - m;
+        const double2* cf = coef + (table_base(m, L) - m) * K;
+        for (int l0 = m; l0 < L; l0 += 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double2 cv[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) cv[k] = cf[(int64_t)(l0 + i) * K + k];
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        double2& acc = (i & 1) ? aO[t][k] : aE[t][k];
+                        acc.x = fma(mu0[t], cv[k].x, acc.x);
+                        acc.y = fma(mu0[t], cv[k].y, acc.y);
+                    }
+                }
+                const double a = al[l0 + i + 1];
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const double nu = fma(x[t] * a, mu0[t], -mu1[t]);
+                    mu1[t] = mu0[t];
+                    mu0[t] = nu;
+                }
+            }
+            rescale<R>(mu0, mu1, kx, anyinact);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        if (j >= NF) continue;
+        const int r = 2 * nside - 1 - j, rp = 2 * nside - 1 + j;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // F_N = E + O, F_S = E - O (zero for rings beyond the polar-skip bound)
+            spec[k * smap + (int64_t)r * ld + m] = cadd(aE[t][k], aO[t][k]);
+            if (j > 0) spec[k * smap + (int64_t)rp * ld + m] = csub(aE[t][k], aO[t][k]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ spin-s kernels
+// All rings, ordered |z| ascending: slot 0 = equator, 2u-1 = north ring 2n-1-u, 2u = south
+// ring 2n-1+u.  Channels per degree: direct (re, im) against G[r][m], mirror (re, im)
+// against G[mirror(r)][-m].
+__device__ inline int slot_ring(int nside, int j) {
+    if (j == 0) return 2 * nside - 1;
+    const int u = (j + 1) / 2;
+    return (j & 1) ? 2 * nside - 1 - u : 2 * nside - 1 + u;
+}
+
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ spec, int64_t ld, int nside,
+                                                   int L, int s, const double* __restrict__ alpha,
+                                                   const double* __restrict__ gamma,
+                                                   const double* __restrict__ smant,
+                                                   const int* __restrict__ sexp,
+                                                   const int* __restrict__ nact,
+                                                   double* __restrict__ partial, int64_t tsize) {
+    constexpr int NV = 16;  // 4 degrees x 4 channels
+    constexpr int T = 64 * NW;
+    constexpr int CH = T * R;
+    __shared__ double lds[2 * NV * T];
+    const int c = blockIdx.x, m = blockIdx.y;
+    const int na = nact[m];
+    if (c * CH >= na) return;
+    const int M = m > s ? m : s;
+    const int nring = 4 * nside - 1;
+
+    double x[R], mu0[R], mu1[R];
+    int kx[R];
+    double2 gD[R], gM[R];
+    const double sm = smant[m];
+    const int se = sexp[m];
+    const int a_exp = m + s, b_exp = m > s ? m - s : s - m;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        mu1[t] = 0.0;
+        if (j < na) {
+            const int r = slot_ring(nside, j);
+            const Ring g = ring_info(nside, r);
+            x[t] = g.z;
+            seed_lane(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
+            gD[t] = spec[(int64_t)r * ld + (L - 1) + m];
+            gM[t] = m ? spec[(int64_t)(nring - 1 - r) * ld + (L - 1) - m] : make_double2(0.0, 0.0);
+        } else {
+            x[t] = 0.0;
+            mu0[t] = 0.0;
+            kx[t] = 0;
+            gD[t] = gM[t] = make_double2(0.0, 0.0);
+        }
+    }
+    bool anyinact = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) anyinact = anyinact || (kx[t] < 0);
+    anyinact = __any(anyinact);
+
+    const double* al = alpha + table_base(m, L) - m;
+    const double* ga = gamma + table_base(m, L) - m;
+    double* part = partial + ((int64_t)c * tsize + table_base(m, L) - m) * 4;
+    int buf = 0;
+    for (int l0 = M; l0 < L; l0 += 4, buf ^= 1) {
+        double acc[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                acc[4 * i + 0] = fma(mu0[t], gD[t].x, acc[4 * i + 0]);
+                acc[4 * i + 1] = fma(mu0[t], gD[t].y, acc[4 * i + 1]);
+                acc[4 * i + 2] = fma(mu0[t], gM[t].x, acc[4 * i + 2]);
+                acc[4 * i + 3] = fma(mu0[t], gM[t].y, acc[4 * i + 3]);
+            }
+            const double a = al[l0 + i + 1], gm_ = ga[l0 + i + 1];
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const double nu = fma(fma(x[t], a, gm_), mu0[t], -mu1[t]);
+                mu1[t] = mu0[t];
+                mu0[t] = nu;
+            }
+        }
+        rescale<R>(mu0, mu1, kx, anyinact);
+        block_reduce<NV, NW>(acc, lds, buf, part + (int64_t)l0 * 4);
+    }
+}
+
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ coef, int nside, int L, int s,
+                                                   const double* __restrict__ alpha,
+                                                   const double* __restrict__ gamma,
+                                                   const double* __restrict__ smant,
+                                                   const int* __restrict__ sexp,
+                                                   const int* __restrict__ nact,
+                                                   double2* __restrict__ spec, int64_t ld) {
+    constexpr int T = 64 * NW;
+    constexpr int CH = T * R;
+    const int c = blockIdx.x, m = blockIdx.y;
+    const int na = nact[m];
+    const int nring = 4 * nside - 1;
+    const int M = m > s ? m : s;
+
+    double x[R], mu0[R], mu1[R];
+    int kx[R];
+    double2 aD[R], aM[R];
+    const double sm = smant[m];
+    const int se = sexp[m];
+    const int a_exp = m + s, b_exp = m > s ? m - s : s - m;
+    bool anyact = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        mu1[t] = 0.0;
+        kx[t] = 0;
+        x[t] = 0.0;
+        mu0[t] = 0.0;
+        aD[t] = aM[t] = make_double2(0.0, 0.0);
+        if (j < na && M < L) {
+            const Ring g = ring_info(nside, slot_ring(nside, j));
+            x[t] = g.z;
+            seed_lane(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
+            anyact = true;
+        }
+    }
+    if (__any(anyact)) {
+        bool anyinact = false;
+#pragma unroll
+        for (int t = 0; t < R; ++t) anyinact = anyinact || (kx[t] < 0);
+        anyinact = __any(anyinact);
+        const double* al = alpha + table_base(m, L) - m;
+        const double* ga = gamma + table_base(m, L) - m;
+        const double2* cf = coef + (table_base(m, L) - m) * 2;
+        for (int l0 = M; l0 < L; l0 += 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const double2 cd = cf[(int64_t)(l0 + i) * 2], cm = cf[(int64_t)(l0 + i) * 2 + 1];
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    aD[t].x = fma(mu0[t], cd.x, aD[t].x);
+                    aD[t].y = fma(mu0[t], cd.y, aD[t].y);
+                    aM[t].x = fma(mu0[t], cm.x, aM[t].x);
+                    aM[t].y = fma(mu0[t], cm.y, aM[t].y);
+                }
+                const double a = al[l0 + i + 1], gm_ = ga[l0 + i + 1];
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    const double nu = fma(fma(x[t], a, gm_), mu0[t], -mu1[t]);
+                    mu1[t] = mu0[t];
+                    mu0[t] = nu;
+                }
+            }
+            rescale<R>(mu0, mu1, kx, anyinact);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        if (j >= nring) continue;
+        const int r = slot_ring(nside, j);
+        spec[(int64_t)r * ld + (L - 1) + m] = aD[t];
+        if (m) spec[(int64_t)(nring - 1 - r) * ld + (L - 1) - m] = aM[t];
+    }
+}
+
+// ------------------------------------------------------------------ tables
+__global__ void k_lat_tables(int L, int s, double* __restrict__ alpha, double* __restrict__ gamma,
+                             double* __restrict__ P) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= L) return;
+    const int M = m > s ? m : s;
+    const int64_t base = table_base(m, L) - m;  // entry l at base + l
+    const int lend = L + GM_TABLE_PAD;
+    for (int l = m; l <= M && l < lend; ++l) {
+        alpha[base + l] = 0.0;
+        if (gamma) gamma[base + l] = 0.0;
+        P[base + l] = 1.0;
+    }
+    double Pm2 = 1.0, Pm1 = 1.0;  // P_{l-2}, P_{l-1}
+    const double mm = (double)m * m, ss = (double)s * s;
+    for (int l = M + 1; l < L; ++l) {
+        const int lp = l - 1;
+        const double lp1 = lp + 1.0;
+        const double K = lp1 * (2.0 * lp + 1.0) / sqrt((lp1 * lp1 - mm) * (lp1 * lp1 - ss));
+        const double A = sqrt((2.0 * lp + 3.0) / (2.0 * lp + 1.0)) * K;
+        const double C = lp > 0 ? A * (double)m * (double)s / ((double)lp * lp1) : 0.0;
+        double B = 0.0;
+        if (lp > 0)
+            B = sqrt((2.0 * lp + 3.0) / (2.0 * lp - 1.0)) * K *
+                sqrt(((double)lp * lp - mm) * ((double)lp * lp - ss)) / ((double)lp * (2.0 * lp + 1.0));
+        const double Pl = (l >= M + 2) ? B * Pm2 : 1.0;
+        alpha[base + l] = A * Pm1 / Pl;
+        if (gamma) gamma[base + l] = C * Pm1 / Pl;
+        P[base + l] = Pl;
+        Pm2 = Pm1;
+        Pm1 = Pl;
+    }
+    for (int l = (M + 1 > L ? M + 1 : L); l < lend; ++l) {
+        alpha[base + l] = 0.0;
+        if (gamma) gamma[base + l] = 0.0;
+        P[base + l] = 0.0;
+    }
+}
+
+// ------------------------------------------------------------------ finish / prep
+// Chunk layout of the analysis kernels: CH rings per workgroup.
+struct LatConfig {
+    int R, NW, CH, nchunks;
+};
+
+__global__ void k_finish_s0(const double2* __restrict__ partial, int K, int nchunk, int CH,
+                            const int* __restrict__ nact, const double* __restrict__ P, int L, int64_t tsize,
+                            double2* __restrict__ alm, int64_t astride, int mode) {
+    const int m = blockIdx.y;
+    const int used = (nact[m] + CH - 1) / CH;
+    const int nu = used < nchunk ? used : nchunk;
+    const int64_t pb = table_base(m, L) - m;
+    const int lmax = L - 1;
+    for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l < L; l += gridDim.x * blockDim.x) {
+        const double p = P[pb + l];
+        for (int k = 0; k < K; ++k) {
+            double2 s = make_double2(0.0, 0.0);
+            for (int c = 0; c < nu; ++c) s = cadd(s, partial[((int64_t)c * tsize + pb + l) * K + k]);
+            s = cscale(s, p);
+            double2& a = alm[k * astride + alm_index(l, m, lmax)];
+            a = mode ? csub(a, s) : s;
+        }
+    }
+}
+
+__global__ void k_prep_s0(const double2* __restrict__ alm, int K, int64_t astride, const double* __restrict__ P,
+                          int L, double2* __restrict__ coef) {
+    const int m = blockIdx.y;
+    const int64_t pb = table_base(m, L) - m;
+    const int lmax = L - 1;
+    for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l < L + GM_TABLE_PAD; l += gridDim.x * blockDim.x) {
+        for (int k = 0; k < K; ++k) {
+            double2 v = make_double2(0.0, 0.0);
+            if (l < L) v = cscale(alm[k * astride + alm_index(l, m, lmax)], P[pb + l]);
+            coef[(pb + l) * K + k] = v;
+        }
+    }
+}
+
+__global__ void k_finish_s(const double2* __restrict__ partial, int nchunk, int CH, const int* __restrict__ nact,
+                           const double* __restrict__ P, int L, int s, int64_t tsize, double2* __restrict__ ae,
+                           double2* __restrict__ ab, int mode) {
+    const int m = blockIdx.y;
+    const int used = (nact[m] + CH - 1) / CH;
+    const int nu = used < nchunk ? used : nchunk;
+    const int64_t pb = table_base(m, L) - m;
+    const int lmax = L - 1;
+    const double sgn_s = (s & 1) ? -1.0 : 1.0;
+    const double sgn_m = (m & 1) ? -1.0 : 1.0;
+    for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l < L; l += gridDim.x * blockDim.x) {
+        double2 E = make_double2(0.0, 0.0), B = E;
+        if (l >= s) {
+            double2 d = make_double2(0.0, 0.0), mi = d;
+            for (int c = 0; c < nu; ++c) {
+                const double2* p = partial + ((int64_t)c * tsize + pb + l) * 2;
+                d = cadd(d, p[0]);
+                mi = cadd(mi, p[1]);
+            }
+            const double pl = P[pb + l];
+            const double2 plus = cscale(d, sgn_s * pl);  // a+_{l,m}
+            double2 am;                                  // a+_{l,-m}
+            if (m) am = cscale(mi, sgn_s * pl * (((l + s) & 1) ? -1.0 : 1.0));
+            else am = plus;
+            const double2 minus = cscale(cconj(am), sgn_m);  // (-1)^m conj a+_{l,-m}
+            E = make_double2(-0.5 * (plus.x + minus.x), -0.5 * (plus.y + minus.y));
+            // B = (i/2)(plus - minus)
+            B = make_double2(-0.5 * (plus.y - minus.y), 0.5 * (plus.x - minus.x));
+        }
+        const int64_t ia = alm_index(l, m, lmax);
+        if (mode) {
+            ae[ia] = csub(ae[ia], E);
+            ab[ia] = csub(ab[ia], B);
+        } else {
+            ae[ia] = E;
+            ab[ia] = B;
+        }
+    }
+}
+
+__global__ void k_prep_s(const double2* __restrict__ ae, const double2* __restrict__ ab,
+                         const double* __restrict__ P, int L, int s, double2* __restrict__ coef) {
+    const int m = blockIdx.y;
+    const int64_t pb = table_base(m, L) - m;
+    const int lmax = L - 1;
+    const double sgn_s = (s & 1) ? -1.0 : 1.0;
+    const double sgn_m = (m & 1) ? -1.0 : 1.0;
+    for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l < L + GM_TABLE_PAD; l += gridDim.x * blockDim.x) {
+        double2 cd = make_double2(0.0, 0.0), cm = cd;
+        if (l < L && l >= s) {
+            const int64_t ia = alm_index(l, m, lmax);
+            const double2 e = ae[ia], b = ab[ia];
+            // a+_{l,m} = -(E + iB); a+_{l,-m} = -(-1)^m (conj E + i conj B)
+            const double2 plus = make_double2(-(e.x - b.y), -(e.y + b.x));
+            const double2 minus = make_double2(-sgn_m * (e.x + b.y), -sgn_m * (-e.y + b.x));
+            const double pl = P[pb + l] * sgn_s;
+            cd = cscale(plus, pl);
+            cm = cscale(minus, pl * (((l + s) & 1) ? -1.0 : 1.0));
+        }
+        coef[(pb + l) * 2] = cd;
+        coef[(pb + l) * 2 + 1] = cm;
+    }
+}
+
+// ------------------------------------------------------------------ configs
+LatConfig lat_config(int64_t nside, int64_t spin) {
+    const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
+    LatConfig cfg;
+    if (nrings <= 128) cfg = {2, 1, 128, 0};
+    else if (nrings <= 512) cfg = {4, 2, 512, 0};
+    else cfg = {8, 4, 2048, 0};
+    cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
+    return cfg;
+}
+
+// synthesis: 4-wave workgroups
+LatConfig syn_config(int64_t nside, int64_t spin) {
+    const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
+    LatConfig cfg;
+    if (nrings <= 128) cfg = {2, 1, 128, 0};
+    else if (nrings <= 512) cfg = {4, 2, 512, 0};
+    else cfg = {8, 4, 2048, 0};
+    cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
+    return cfg;
+}
+
+int check_lat(int64_t nside, int64_t lmax, int64_t spin) {
+    if (nside < 1 || (nside & (nside - 1))) return fail(GM_E_ARG, "nside must be a power of 2");
+    if (nside > 8192) return fail(GM_E_UNSUPPORTED, "nside > 8192 not supported");
+    if (lmax < 0 || lmax > 4 * nside * 6) return fail(GM_E_ARG, "bad lmax %lld", (long long)lmax);
+    if (spin < 0 || spin > lmax) return fail(GM_E_ARG, "spin must satisfy 0 <= spin <= lmax");
+    return GM_OK;
+}
+
+}  // namespace
+}  // namespace gm
+
+using namespace gm;
+
+extern "C" int64_t gm_table_size(int64_t lmax) { return table_size(lmax + 1); }
+
+extern "C" int64_t gm_lat_nchunks(int64_t nside, int64_t spin) { return lat_config(nside, spin).nchunks; }
+
+extern "C" int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* gamma, double* P, void* stream) {
+    if (lmax < 0 || spin < 0 || spin > lmax) return fail(GM_E_ARG, "gm_lat_tables: bad lmax / spin");
+    if (!alpha || !P || (spin && !gamma)) return fail(GM_E_ARG, "gm_lat_tables: null pointer");
+    const int L = (int)lmax + 1;
+    hipLaunchKernelGGL(k_lat_tables, dim3((L + 63) / 64), dim3(64), 0, (hipStream_t)stream, L, (int)spin, alpha,
+                       gamma, P);
+    return check_launch("gm_lat_tables");
+}
+
+#define GM_LAUNCH_CFG(KERNEL_T, cfg, ...)                                                        \
+    do {                                                                                         \
+        if ((cfg).R == 2) hipLaunchKernelGGL((KERNEL_T<2, 1>), __VA_ARGS__);                     \
+        else if ((cfg).R == 4) hipLaunchKernelGGL((KERNEL_T<4, 2>), __VA_ARGS__);                \
+        else hipLaunchKernelGGL((KERNEL_T<8, 4>), __VA_ARGS__);                                  \
+    } while (0)
+
+extern "C" int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
+                             int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
+                             const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+    if (int e = check_lat(nside, lmax, 0)) return e;
+    if (!spec || !alpha || !seed_mant || !seed_exp || !nactive || !partial)
+        return fail(GM_E_ARG, "gm_lat_ana_s0: null pointer");
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_ana_s0: nmaps must be 1 or 2");
+    const int64_t L = lmax + 1;
+    if (ld < L) return fail(GM_E_SHAPE, "gm_lat_ana_s0: ld < lmax + 1");
+    if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "gm_lat_ana_s0: map stride too small");
+    const LatConfig cfg = lat_config(nside, 0);
+    if (nchunks != cfg.nchunks)
+        return fail(GM_E_SHAPE, "gm_lat_ana_s0: partial has %lld chunks, need %d", (long long)nchunks, cfg.nchunks);
+    const int64_t ts = table_size(L);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* sp = reinterpret_cast<const double2*>(spec);
+    if (nmaps == 1) {
+        if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s0<1, 2, 1>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s0<1, 4, 2>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        else hipLaunchKernelGGL((k_ana_s0<1, 8, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+    } else {
+        if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s0<2, 2, 1>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s0<2, 4, 2>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        else hipLaunchKernelGGL((k_ana_s0<2, 8, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+    }
+    return check_launch("gm_lat_ana_s0");
+}
+
+extern "C" int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
+                                const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
+                                void* stream) {
+    if (int e = check_lat(nside, lmax, 0)) return e;
+    if (!partial || !nactive || !P || !alm) return fail(GM_E_ARG, "gm_lat_finish_s0: null pointer");
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_finish_s0: nmaps must be 1 or 2");
+    const int64_t L = lmax + 1, nelem = L * (L + 1) / 2;
+    if (nmaps > 1 && alm_map_stride < nelem) return fail(GM_E_SHAPE, "gm_lat_finish_s0: alm stride too small");
+    const LatConfig cfg = lat_config(nside, 0);
+    hipLaunchKernelGGL(k_finish_s0, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const double2*>(partial), (int)nmaps, cfg.nchunks, cfg.CH, nactive, P, (int)L,
+                       table_size(L), reinterpret_cast<double2*>(alm), alm_map_stride, mode);
+    return check_launch("gm_lat_finish_s0");
+}
+
+extern "C" int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_stride, const double* P,
+                              int64_t lmax, double* coef, void* stream) {
+    if (lmax < 0 || !alm || !P || !coef) return fail(GM_E_ARG, "gm_lat_prep_s0: bad arguments");
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_prep_s0: nmaps must be 1 or 2");
+    const int64_t L = lmax + 1;
+    hipLaunchKernelGGL(k_prep_s0, dim3((unsigned)((L + GM_TABLE_PAD + 255) / 256), (unsigned)L), dim3(256), 0,
+                       (hipStream_t)stream, reinterpret_cast<const double2*>(alm), (int)nmaps, alm_map_stride, P,
+                       (int)L, reinterpret_cast<double2*>(coef));
+    return check_launch("gm_lat_prep_s0");
+}
+
+extern "C" int gm_lat_syn_s0(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
+                             const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, double* spec,
+                             int64_t ld, int64_t smap, void* stream) {
+    if (int e = check_lat(nside, lmax, 0)) return e;
+    if (!coef || !alpha || !seed_mant || !seed_exp || !nactive || !spec)
+        return fail(GM_E_ARG, "gm_lat_syn_s0: null pointer");
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_syn_s0: nmaps must be 1 or 2");
+    const int64_t L = lmax + 1;
+    if (ld < L) return fail(GM_E_SHAPE, "gm_lat_syn_s0: ld < lmax + 1");
+    if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "gm_lat_syn_s0: map stride too small");
+    const LatConfig cfg = syn_config(nside, 0);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* cf = reinterpret_cast<const double2*>(coef);
+    double2* sp = reinterpret_cast<double2*>(spec);
+    if (nmaps == 1) {
+        if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s0<1, 2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+        else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s0<1, 4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+        else hipLaunchKernelGGL((k_syn_s0<1, 8, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+    } else {
+        if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s0<2, 2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+        else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s0<2, 4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+        else hipLaunchKernelGGL((k_syn_s0<2, 8, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+    }
+    return check_launch("gm_lat_syn_s0");
+}
+
+extern "C" int gm_lat_ana_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
+                            const double* alpha, const double* gamma, const double* seed_mant, const int32_t* seed_exp,
+                            const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+    if (int e = check_lat(nside, lmax, spin)) return e;
+    if (spin == 0) return fail(GM_E_ARG, "gm_lat_ana_s: use gm_lat_ana_s0 for spin 0");
+    if (!spec || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !partial)
+        return fail(GM_E_ARG, "gm_lat_ana_s: null pointer");
+    const int64_t L = lmax + 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_lat_ana_s: ld < 2 lmax + 1");
+    const LatConfig cfg = lat_config(nside, spin);
+    if (nchunks != cfg.nchunks)
+        return fail(GM_E_SHAPE, "gm_lat_ana_s: partial has %lld chunks, need %d", (long long)nchunks, cfg.nchunks);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* sp = reinterpret_cast<const double2*>(spec);
+    const int64_t ts = table_size(L);
+    if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s<2, 1>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts);
+    else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s<4, 2>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts);
+    else hipLaunchKernelGGL((k_ana_s<8, 4>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts);
+    return check_launch("gm_lat_ana_s");
+}
+
+extern "C" int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
+                               int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream) {
+    if (int e = check_lat(nside, lmax, spin)) return e;
+    if (!partial || !nactive || !P || !alm_e || !alm_b) return fail(GM_E_ARG, "gm_lat_finish_s: null pointer");
+    const int64_t L = lmax + 1;
+    const LatConfig cfg = lat_config(nside, spin);
+    hipLaunchKernelGGL(k_finish_s, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const double2*>(partial), cfg.nchunks, cfg.CH, nactive, P, (int)L, (int)spin,
+                       table_size(L), reinterpret_cast<double2*>(alm_e), reinterpret_cast<double2*>(alm_b), mode);
+    return check_launch("gm_lat_finish_s");
+}
+
+extern "C" int gm_lat_prep_s(const double* alm_e, const double* alm_b, const double* P, int64_t lmax, int64_t spin,
+                             double* coef, void* stream) {
+    if (lmax < 0 || spin < 0 || spin > lmax || !alm_e || !alm_b || !P || !coef)
+        return fail(GM_E_ARG, "gm_lat_prep_s: bad arguments");
+    const int64_t L = lmax + 1;
+    hipLaunchKernelGGL(k_prep_s, dim3((unsigned)((L + GM_TABLE_PAD + 255) / 256), (unsigned)L), dim3(256), 0,
+                       (hipStream_t)stream, reinterpret_cast<const double2*>(alm_e),
+                       reinterpret_cast<const double2*>(alm_b), P, (int)L, (int)spin,
+                       reinterpret_cast<double2*>(coef));
+    return check_launch("gm_lat_prep_s");
+}
+
+extern "C" int gm_lat_syn_s(const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
+                            const double* gamma, const double* seed_mant, const int32_t* seed_exp,
+                            const int32_t* nactive, double* spec, int64_t ld, void* stream) {
+    if (int e = check_lat(nside, lmax, spin)) return e;
+    if (spin == 0) return fail(GM_E_ARG, "gm_lat_syn_s: use gm_lat_syn_s0 for spin 0");
+    if (!coef || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !spec)
+        return fail(GM_E_ARG, "gm_lat_syn_s: null pointer");
+    const int64_t L = lmax + 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_lat_syn_s: ld < 2 lmax + 1");
+    const LatConfig cfg = syn_config(nside, spin);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* cf = reinterpret_cast<const double2*>(coef);
+    double2* sp = reinterpret_cast<double2*>(spec);
+    if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s<2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);
+    else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s<4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);
+    else hipLaunchKernelGGL((k_syn_s<8, 4>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);
+    return check_launch("gm_lat_syn_s");
+}

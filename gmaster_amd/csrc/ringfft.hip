@@ -1,0 +1,607 @@
+// Ring (azimuthal) stage of the HEALPix transforms on gfx950, FP64.
+//
+// Each HEALPix ring has nphi = 4 i (polar caps) or 4 nside (belt) pixels, so every
+// length is even: a real ring of length n is transformed as one complex FFT of
+// N = n / 2 points held entirely in LDS (N <= 8192 -> 128 KiB at Nside 4096), mixed
+// radix 4/2/3/5 Stockham passes plus a direct "DFT pass" (Horner over the p inputs, one
+// output per thread) for larger prime factors.  One workgroup per ring; rings are
+// launched in size buckets so the dynamic LDS matches the largest ring of the bucket.
+//
+// The forward output is the *phased* spectrum  G[r][m] = w e^{-i m phi0} X[m mod n]
+// (w = 4 pi / npix), exactly the right-hand side of the Legendre analysis, so no
+// separate weight / phase pass runs.  The Jacobi residual is formed in this space by
+// the fold kernel at the end of the file, with no FFT inside the refinement loop.
+#include <stdarg.h>
+
+#include "common.h"
+
+namespace gm {
+
+namespace {
+
+__device__ inline double2 twiddle(int64_t num, int64_t den, double sign) {
+    // e^{sign 2 pi i num / den}
+    double s, c;
+    sincospi(sign * 2.0 * (double)num / (double)den, &s, &c);
+    return make_double2(c, s);
+}
+
+template <int P>
+__device__ inline void dft_small(double2* v, double sign) {
+    if constexpr (P == 2) {
+        const double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (P == 4) {
+        const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const double2 t2 = cadd(v[1], v[3]);
+        const double2 d = csub(v[1], v[3]);
+        const double2 t3 = make_double2(-sign * d.y, sign * d.x);  // d * (sign i)
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = cadd(t1, t3);
+        v[3] = csub(t1, t3);
+    } else if constexpr (P == 3) {
+        const double h = 0.86602540378443864676;  // sqrt(3)/2
+        const double2 t = cadd(v[1], v[2]);
+        const double2 d = csub(v[1], v[2]);
+        const double2 u = make_double2(-sign * h * d.y, sign * h * d.x);
+        const double2 base = make_double2(v[0].x - 0.5 * t.x, v[0].y - 0.5 * t.y);
+        v[0] = cadd(v[0], t);
+        v[1] = cadd(base, u);
+        v[2] = csub(base, u);
+    } else {  // P == 5: direct 5-point DFT with exact constants
+        const double c1 = 0.30901699437494742410, s1 = 0.95105651629515357212;
+        const double c2 = -0.80901699437494742410, s2 = 0.58778525229247312917;
+        const double2 w[5] = {make_double2(1.0, 0.0), make_double2(c1, sign * s1),
+                              make_double2(c2, sign * s2), make_double2(c2, -sign * s2),
+                              make_double2(c1, -sign * s1)};
+        double2 out[5];
+        for (int k = 0; k < 5; ++k) {
+            double2 acc = v[0];
+            for (int q = 1; q < 5; ++q) acc = cadd(acc, cmul(v[q], w[(q * k) % 5]));
+            out[k] = acc;
+        }
+        for (int k = 0; k < 5; ++k) v[k] = out[k];
+    }
+}
+
+// One Stockham pass of radix P (P in {2,3,4,5}) over buf[0..N), in place.  With
+// blockDim.x >= N / 8 each thread owns at most ceil(8 / P) butterflies.
+template <int P>
+__device__ void radix_pass(double2* buf, int N, int Ns, double sign) {
+    const int J = N / P;
+    const int T = blockDim.x;
+    constexpr int MB = (8 + P - 1) / P;
+    double2 v[MB][P];
+#pragma unroll
+    for (int t = 0; t < MB; ++t) {
+        const int j = threadIdx.x + t * T;
+        if (j < J) {
+            const int k = j % Ns;
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[t][q] = buf[j + q * J];
+            if (Ns > 1) {
+                const double2 w1 = twiddle(k, (int64_t)Ns * P, sign);
+                double2 w = w1;
+#pragma unroll
+                for (int q = 1; q < P; ++q) {
+                    v[t][q] = cmul(v[t][q], w);
+                    w = cmul(w, w1);
+                }
+            }
+            dft_small<P>(v[t], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MB; ++t) {
+        const int j = threadIdx.x + t * T;
+        if (j < J) {
+            const int k = j % Ns;
+            const int base = (j / Ns) * Ns * P + k;
+#pragma unroll
+            for (int q = 0; q < P; ++q) buf[base + q * Ns] = v[t][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Direct DFT pass of any radix p: each thread computes whole outputs by Horner's rule
+// (at most 8 outputs per thread with blockDim.x >= N / 8).
+__device__ void dft_pass(double2* buf, int N, int p, int Ns, double sign) {
+    const int J = N / p;
+    const int T = blockDim.x;
+    double2 out[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int f = threadIdx.x + t * T;
+        if (f < N) {
+            const int j = f % J, q = f / J;
+            const int k = j % Ns;
+            const double2 w = twiddle(k + (int64_t)q * Ns, (int64_t)Ns * p, sign);
+            double2 acc = buf[j + (p - 1) * J];
+            for (int u = p - 2; u >= 0; --u) acc = cadd(cmul(acc, w), buf[j + u * J]);
+            out[t] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int f = threadIdx.x + t * T;
+        if (f < N) {
+            const int j = f % J, q = f / J;
+            const int k = j % Ns;
+            buf[(j / Ns) * Ns * p + k + q * Ns] = out[t];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ int smallest_factor(int n) {
+    if (n % 2 == 0) return 2;
+    for (int f = 3; f * f <= n; f += 2)
+        if (n % f == 0) return f;
+    return n;
+}
+
+// In-place complex FFT of length N in LDS, sign -1 forward, +1 inverse (unnormalised).
+// Requires blockDim.x >= N / 8 (at most 8 outputs per thread per pass).
+__device__ void fft_lds(double2* buf, int N, double sign) {
+    int Ns = 1, rem = N;
+    while (rem > 1) {
+        int p;
+        if (rem % 4 == 0) p = 4;
+        else if (rem % 2 == 0) p = 2;
+        else if (rem % 3 == 0) p = 3;
+        else if (rem % 5 == 0) p = 5;
+        else p = smallest_factor(rem);
+        switch (p) {
+            case 4: radix_pass<4>(buf, N, Ns, sign); break;
+            case 2: radix_pass<2>(buf, N, Ns, sign); break;
+            case 3: radix_pass<3>(buf, N, Ns, sign); break;
+            case 5: radix_pass<5>(buf, N, Ns, sign); break;
+            default: dft_pass(buf, N, p, Ns, sign); break;
+        }
+        Ns *= p;
+        rem /= p;
+    }
+}
+
+// Ring index of block b within a size bucket: caps take north indices [qlo, qhi] in both
+// hemispheres; the belt bucket (qlo == 0) takes rings nside-1 .. 3 nside-1.
+__device__ inline int bucket_ring(int nside, int qlo, int qhi, int b) {
+    if (qlo == 0) return nside - 1 + b;
+    const int count = qhi - qlo + 1;
+    if (b < count) return qlo + b - 1;
+    const int q = qlo + (b - count);
+    return 4 * nside - 1 - q;
+}
+
+// z[k] = x[2k] + i x[2k+1] -> X[k] (k = 0..N) in place; X[N] returned through *xn.
+__device__ void r2c_post(double2* buf, int N, double2* xn) {
+    const int n = 2 * N;
+    for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
+        if (k == 0) {
+            const double2 z = buf[0];
+            buf[0] = make_double2(z.x + z.y, 0.0);
+            *xn = make_double2(z.x - z.y, 0.0);
+            continue;
+        }
+        const int kk = N - k;
+        const double2 a = buf[k], b = buf[kk];
+        // X[k] = (a + conj b)/2 - (i/2) e^{-2 pi i k / n} (a - conj b)
+        const double2 e = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+        const double2 o = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y + b.y));
+        const double2 w = twiddle(k, n, -1.0);
+        const double2 wo = cmul(w, o);
+        const double2 xk = make_double2(e.x + wo.y, e.y - wo.x);
+        if (kk != k) {
+            const double2 e2 = make_double2(e.x, -e.y);   // (b + conj a)/2
+            const double2 o2 = make_double2(-o.x, o.y);   // (b - conj a)/2
+            const double2 w2 = twiddle(kk, n, -1.0);
+            const double2 wo2 = cmul(w2, o2);
+            buf[kk] = make_double2(e2.x + wo2.y, e2.y - wo2.x);
+        }
+        buf[k] = xk;
+    }
+    __syncthreads();
+}
+
+// X[k] (k = 0..N, X[N] in xn) -> Z[k] for the inverse packed FFT, in place.
+__device__ void c2r_pre(double2* buf, int N, double2 xn) {
+    const int n = 2 * N;
+    for (int k = threadIdx.x; k <= N / 2; k += blockDim.x) {
+        const int kk = N - k;
+        const double2 a = buf[k];
+        const double2 b = (k == 0) ? xn : buf[kk];
+        // Z[k] = (X_k + conj X_{N-k}) + i e^{2 pi i k/n} (X_k - conj X_{N-k})
+        const double2 s = make_double2(a.x + b.x, a.y - b.y);
+        const double2 d = make_double2(a.x - b.x, a.y + b.y);
+        const double2 wd = cmul(twiddle(k, n, 1.0), d);
+        const double2 zk = make_double2(s.x - wd.y, s.y + wd.x);
+        if (k != 0 && kk != k) {
+            const double2 s2 = make_double2(s.x, -s.y);
+            const double2 d2 = make_double2(-d.x, d.y);
+            const double2 wd2 = cmul(twiddle(kk, n, 1.0), d2);
+            buf[kk] = make_double2(s2.x - wd2.y, s2.y + wd2.x);
+        }
+        buf[k] = zk;
+    }
+    __syncthreads();
+}
+
+__device__ inline double2 half_spectrum(const double2* buf, double2 xn, int N, int kk) {
+    // X[kk] for kk in [0, 2N) from the half spectrum X[0..N]
+    if (kk < N) return buf[kk];
+    if (kk == N) return xn;
+    return cconj(buf[2 * N - kk]);
+}
+
+// ----------------------------------------------------------------------- forward kernels
+__global__ void k_ring_fwd_s0(const double* __restrict__ maps, int64_t map_stride, int nside,
+                              int L, int qlo, int qhi, double2* __restrict__ spec, int64_t ld,
+                              int64_t spec_map_stride) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2;
+    const double* x = maps + blockIdx.y * map_stride + g.offset;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) buf[k] = make_double2(x[2 * k], x[2 * k + 1]);
+    __syncthreads();
+    fft_lds(buf, N, -1.0);
+    r2c_post(buf, N, &xn);
+    const double w = 4.0 * M_PI / (12.0 * (double)nside * (double)nside);
+    double2* out = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
+    for (int m = threadIdx.x; m < L; m += blockDim.x) {
+        const double2 X = half_spectrum(buf, xn, N, m % n);
+        const double2 ph = phase_pi(m, g.phi0_num, g.phi0_den, -1.0);
+        out[m] = cscale(cmul(ph, X), w);
+    }
+}
+
+__global__ void k_ring_fwd_s(const double* __restrict__ q, const double* __restrict__ u, int nside,
+                             int L, int qlo, int qhi, double2* __restrict__ spec, int64_t ld) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2;
+    const double w = 4.0 * M_PI / (12.0 * (double)nside * (double)nside);
+    double2* out = spec + (int64_t)r * ld;
+    const int W = 2 * L - 1;
+    for (int comp = 0; comp < 2; ++comp) {
+        const double* x = (comp == 0 ? q : u) + g.offset;
+        for (int k = threadIdx.x; k < N; k += blockDim.x) buf[k] = make_double2(x[2 * k], x[2 * k + 1]);
+        __syncthreads();
+        fft_lds(buf, N, -1.0);
+        r2c_post(buf, N, &xn);
+        for (int j = threadIdx.x; j < W; j += blockDim.x) {
+            const int m = j - (L - 1);
+            int kk = m % n;
+            if (kk < 0) kk += n;
+            const double2 X = half_spectrum(buf, xn, N, kk);
+            const double2 v = cscale(cmul(phase_pi(m, g.phi0_num, g.phi0_den, -1.0), X), w);
+            if (comp == 0) {
+                out[j] = v;
+            } else {
+                const double2 o = out[j];  // written by this same thread in pass 0
+                out[j] = make_double2(o.x - v.y, o.y + v.x);  // + i v
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------- inverse kernels
+// Y(k) = sum_{m = k mod n, m in [mlo, mhi]} F[m] e^{i m phi0}; F indexed at m - mlo_index.
+__device__ inline double2 fold_sum(const double2* F, int64_t col0, int mlo, int mhi, int k, int n,
+                                   const Ring& g) {
+    // first m >= mlo with m = k (mod n)
+    int64_t t = ((int64_t)k - mlo) % n;
+    if (t < 0) t += n;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t m = mlo + t; m <= mhi; m += n)
+        acc = cadd(acc, cmul(F[m + col0], phase_pi(m, g.phi0_num, g.phi0_den, 1.0)));
+    return acc;
+}
+
+__global__ void k_ring_inv_s0(const double2* __restrict__ spec, int64_t ld, int64_t spec_map_stride,
+                              int nside, int L, int qlo, int qhi, double* __restrict__ maps,
+                              int64_t map_stride) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2;
+    const double2* F = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
+    for (int k = threadIdx.x; k <= N; k += blockDim.x) {
+        // X[k] = sum_{m=k} F e^{i m phi0} + conj(sum_{m=-k, m>=1} F e^{i m phi0})
+        const double2 a = fold_sum(F, 0, 0, L - 1, k, n, g);
+        const int kneg = (n - k) % n;
+        const double2 b = fold_sum(F, 0, 1, L - 1, kneg, n, g);
+        const double2 X = make_double2(a.x + b.x, a.y - b.y);
+        if (k < N) buf[k] = X;
+        else xn = X;
+    }
+    __syncthreads();
+    c2r_pre(buf, N, xn);
+    fft_lds(buf, N, 1.0);
+    double* x = maps + blockIdx.y * map_stride + g.offset;
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        const double2 z = buf[k];
+        x[2 * k] = z.x;
+        x[2 * k + 1] = z.y;
+    }
+}
+
+__global__ void k_ring_inv_s(const double2* __restrict__ spec, int64_t ld, int nside, int L, int qlo,
+                             int qhi, double* __restrict__ q, double* __restrict__ u) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2;
+    const double2* F = spec + (int64_t)r * ld;
+    const int64_t col0 = L - 1;  // F[m] lives at column m + L - 1
+    // XQ = (Y(k) + conj Y(-k)) / 2 and XU = -(i/2)(Y(k) - conj Y(-k)); the folds are cheap
+    // next to the FFT, so each component refolds instead of holding the other in registers.
+    for (int comp = 0; comp < 2; ++comp) {
+        for (int k = threadIdx.x; k <= N; k += blockDim.x) {
+            const double2 yp = fold_sum(F, col0, -(L - 1), L - 1, k, n, g);
+            const double2 ym = fold_sum(F, col0, -(L - 1), L - 1, (n - k) % n, n, g);
+            double2 X;
+            if (comp == 0) {
+                X = make_double2(0.5 * (yp.x + ym.x), 0.5 * (yp.y - ym.y));
+            } else {
+                const double2 d = make_double2(yp.x - ym.x, yp.y + ym.y);
+                X = make_double2(0.5 * d.y, -0.5 * d.x);
+            }
+            if (k < N) buf[k] = X;
+            else xn = X;
+        }
+        __syncthreads();
+        c2r_pre(buf, N, xn);
+        fft_lds(buf, N, 1.0);
+        double* x = (comp == 0 ? q : u) + g.offset;
+        for (int k = threadIdx.x; k < N; k += blockDim.x) {
+            const double2 z = buf[k];
+            x[2 * k] = z.x;
+            x[2 * k + 1] = z.y;
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------- fold residual
+// One workgroup per ring, one thread per residue class c of m modulo nphi.
+__global__ void k_fold_s0(const double2* __restrict__ F, const double2* G, double2* R, int64_t ld,
+                          int64_t map_stride, int nside, int L) {
+    const int r = blockIdx.x;
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi;
+    const double sigma = g.phi0_num ? -1.0 : 1.0;
+    const double scale = 4.0 * M_PI / (12.0 * (double)nside * (double)nside) * (double)n;
+    const double2* f = F + blockIdx.y * map_stride + (int64_t)r * ld;
+    const double2* gin = G + blockIdx.y * map_stride + (int64_t)r * ld;
+    double2* out = R + blockIdx.y * map_stride + (int64_t)r * ld;
+    const int nres = n < L ? n : L;
+    for (int c = threadIdx.x; c < nres; c += blockDim.x) {
+        double2 s = make_double2(0.0, 0.0);
+        // m = c + j n over (-L, L); F[-m] = conj F[m]
+        int64_t jlo = -(((int64_t)L - 1 + c) / n);
+        for (int64_t j = jlo; c + j * n < L; ++j) {
+            const int64_t m = c + j * n;
+            double2 v = (m >= 0) ? f[m] : cconj(f[-m]);
+            if (sigma < 0 && (j & 1)) v = make_double2(-v.x, -v.y);
+            s = cadd(s, v);
+        }
+        s = cscale(s, scale);
+        for (int64_t j = 0; c + j * n < L; ++j) {
+            const int64_t k = c + j * n;
+            const double sg = (sigma < 0 && (j & 1)) ? -1.0 : 1.0;
+            const double2 gk = gin[k];
+            out[k] = make_double2(sg * s.x - gk.x, sg * s.y - gk.y);
+        }
+    }
+}
+
+__global__ void k_fold_s(const double2* __restrict__ F, const double2* G, double2* R, int64_t ld,
+                         int nside, int L) {
+    const int r = blockIdx.x;
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi;
+    const double sigma = g.phi0_num ? -1.0 : 1.0;
+    const double scale = 4.0 * M_PI / (12.0 * (double)nside * (double)nside) * (double)n;
+    const int64_t col0 = L - 1;
+    const double2* f = F + (int64_t)r * ld + col0;
+    const double2* gin = G + (int64_t)r * ld + col0;
+    double2* out = R + (int64_t)r * ld + col0;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        // all m = c + j n with |m| < L
+        const int64_t jlo = -(((int64_t)L - 1 + c) / n);
+        if (c + jlo * n >= L) continue;
+        double2 s = make_double2(0.0, 0.0);
+        for (int64_t j = jlo; c + j * n < L; ++j) {
+            double2 v = f[c + j * n];
+            if (sigma < 0 && (j & 1)) v = make_double2(-v.x, -v.y);
+            s = cadd(s, v);
+        }
+        s = cscale(s, scale);
+        for (int64_t j = jlo; c + j * n < L; ++j) {
+            const int64_t k = c + j * n;
+            const double sg = (sigma < 0 && (j & 1)) ? -1.0 : 1.0;
+            const double2 gk = gin[k];
+            out[k] = make_double2(sg * s.x - gk.x, sg * s.y - gk.y);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------- launch helpers
+struct Bucket {
+    int qlo, qhi;  // caps: north indices; belt: qlo == 0
+    int nblocks;
+    int N;         // largest N = nphi / 2 in the bucket
+};
+
+// Size buckets: caps with N = 2q in (2^(b-1), 2^b], then the belt.
+int make_buckets(int nside, Bucket* out) {
+    int nb = 0;
+    int q = 1;
+    while (q <= nside - 1) {
+        int N = 2 * q;
+        int pow2 = 1;
+        while (pow2 < N) pow2 *= 2;
+        int qhi = pow2 / 2;
+        if (qhi > nside - 1) qhi = nside - 1;
+        out[nb].qlo = q;
+        out[nb].qhi = qhi;
+        out[nb].nblocks = 2 * (qhi - q + 1);
+        out[nb].N = 2 * qhi;
+        ++nb;
+        q = qhi + 1;
+    }
+    out[nb].qlo = 0;
+    out[nb].qhi = 0;
+    out[nb].nblocks = 2 * nside + 1;
+    out[nb].N = 2 * nside;
+    return nb + 1;
+}
+
+inline int threads_for(int N) {
+    int t = N / 8;
+    if (t < 64) t = 64;
+    t = (t + 63) / 64 * 64;
+    if (t > 1024) t = 1024;
+    return t;
+}
+
+constexpr int kMaxNside = 4096;
+
+template <typename K>
+int set_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return fail(GM_E_LAUNCH, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    }
+    return GM_OK;
+}
+
+int check_ring_args(int64_t nside, int64_t lmax) {
+    if (nside < 1 || (nside & (nside - 1)) != 0) return fail(GM_E_ARG, "nside must be a power of 2 (got %lld)", (long long)nside);
+    if (nside > kMaxNside) return fail(GM_E_UNSUPPORTED, "ring FFT supports nside <= %d (got %lld)", kMaxNside, (long long)nside);
+    if (lmax < 0) return fail(GM_E_ARG, "lmax must be >= 0");
+    return GM_OK;
+}
+
+}  // namespace
+
+}  // namespace gm
+
+using namespace gm;
+
+extern "C" int gm_ring_fft_fwd_s0(const double* maps, int64_t nmaps, int64_t map_stride, int64_t nside,
+                                  int64_t lmax, double* spec, int64_t ld, int64_t spec_map_stride,
+                                  void* stream) {
+    if (int e = check_ring_args(nside, lmax)) return e;
+    if (!maps || !spec) return fail(GM_E_ARG, "gm_ring_fft_fwd_s0: null pointer");
+    const int64_t L = lmax + 1, npix = 12 * nside * nside, nring = 4 * nside - 1;
+    if (nmaps < 1) return fail(GM_E_ARG, "gm_ring_fft_fwd_s0: nmaps must be >= 1");
+    if (ld < L) return fail(GM_E_SHAPE, "gm_ring_fft_fwd_s0: ld (%lld) < lmax + 1", (long long)ld);
+    if (nmaps > 1 && (map_stride < npix || spec_map_stride < nring * ld))
+        return fail(GM_E_SHAPE, "gm_ring_fft_fwd_s0: map strides too small");
+    Bucket b[32];
+    const int nb = make_buckets((int)nside, b);
+    for (int i = 0; i < nb; ++i) {
+        const size_t lds = (size_t)b[i].N * sizeof(double2);
+        if (int e = set_lds(k_ring_fwd_s0, lds)) return e;
+        hipLaunchKernelGGL(k_ring_fwd_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
+                           (hipStream_t)stream, maps, map_stride, (int)nside, (int)L, b[i].qlo, b[i].qhi,
+                           reinterpret_cast<double2*>(spec), ld, spec_map_stride);
+    }
+    return check_launch("gm_ring_fft_fwd_s0");
+}
+
+extern "C" int gm_ring_fft_fwd_s(const double* q, const double* u, int64_t nside, int64_t lmax, double* spec,
+                                 int64_t ld, void* stream) {
+    if (int e = check_ring_args(nside, lmax)) return e;
+    if (!q || !u || !spec) return fail(GM_E_ARG, "gm_ring_fft_fwd_s: null pointer");
+    const int64_t L = lmax + 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_ring_fft_fwd_s: ld (%lld) < 2 lmax + 1", (long long)ld);
+    Bucket b[32];
+    const int nb = make_buckets((int)nside, b);
+    for (int i = 0; i < nb; ++i) {
+        const size_t lds = (size_t)b[i].N * sizeof(double2);
+        if (int e = set_lds(k_ring_fwd_s, lds)) return e;
+        hipLaunchKernelGGL(k_ring_fwd_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
+                           q, u, (int)nside, (int)L, b[i].qlo, b[i].qhi, reinterpret_cast<double2*>(spec), ld);
+    }
+    return check_launch("gm_ring_fft_fwd_s");
+}
+
+extern "C" int gm_ring_fft_inv_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t spec_map_stride,
+                                  int64_t nside, int64_t lmax, double* maps, int64_t map_stride, void* stream) {
+    if (int e = check_ring_args(nside, lmax)) return e;
+    if (!maps || !spec) return fail(GM_E_ARG, "gm_ring_fft_inv_s0: null pointer");
+    const int64_t L = lmax + 1, npix = 12 * nside * nside, nring = 4 * nside - 1;
+    if (nmaps < 1) return fail(GM_E_ARG, "gm_ring_fft_inv_s0: nmaps must be >= 1");
+    if (ld < L) return fail(GM_E_SHAPE, "gm_ring_fft_inv_s0: ld < lmax + 1");
+    if (nmaps > 1 && (map_stride < npix || spec_map_stride < nring * ld))
+        return fail(GM_E_SHAPE, "gm_ring_fft_inv_s0: map strides too small");
+    Bucket b[32];
+    const int nb = make_buckets((int)nside, b);
+    for (int i = 0; i < nb; ++i) {
+        const size_t lds = (size_t)b[i].N * sizeof(double2);
+        if (int e = set_lds(k_ring_inv_s0, lds)) return e;
+        hipLaunchKernelGGL(k_ring_inv_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
+                           (hipStream_t)stream, reinterpret_cast<const double2*>(spec), ld, spec_map_stride,
+                           (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride);
+    }
+    return check_launch("gm_ring_fft_inv_s0");
+}
+
+extern "C" int gm_ring_fft_inv_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, double* q,
+                                 double* u, void* stream) {
+    if (int e = check_ring_args(nside, lmax)) return e;
+    if (!spec || !q || !u) return fail(GM_E_ARG, "gm_ring_fft_inv_s: null pointer");
+    const int64_t L = lmax + 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_ring_fft_inv_s: ld < 2 lmax + 1");
+    Bucket b[32];
+    const int nb = make_buckets((int)nside, b);
+    for (int i = 0; i < nb; ++i) {
+        const size_t lds = (size_t)b[i].N * sizeof(double2);
+        if (int e = set_lds(k_ring_inv_s, lds)) return e;
+        hipLaunchKernelGGL(k_ring_inv_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
+                           reinterpret_cast<const double2*>(spec), ld, (int)nside, (int)L, b[i].qlo, b[i].qhi,
+                           q, u);
+    }
+    return check_launch("gm_ring_fft_inv_s");
+}
+
+extern "C" int gm_ring_fold_residual(const double* F, const double* G, double* R, int64_t nmaps, int64_t ld,
+                                     int64_t map_stride, int64_t nside, int64_t lmax, void* stream) {
+    if (nside < 1 || lmax < 0) return fail(GM_E_ARG, "gm_ring_fold_residual: bad nside / lmax");
+    if (!F || !G || !R) return fail(GM_E_ARG, "gm_ring_fold_residual: null pointer");
+    const int64_t L = lmax + 1, nring = 4 * nside - 1;
+    if (ld < L) return fail(GM_E_SHAPE, "gm_ring_fold_residual: ld < lmax + 1");
+    if (nmaps < 1 || (nmaps > 1 && map_stride < nring * ld))
+        return fail(GM_E_SHAPE, "gm_ring_fold_residual: bad nmaps / map_stride");
+    hipLaunchKernelGGL(k_fold_s0, dim3((unsigned)nring, (unsigned)nmaps), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const double2*>(F), reinterpret_cast<const double2*>(G),
+                       reinterpret_cast<double2*>(R), ld, map_stride, (int)nside, (int)L);
+    return check_launch("gm_ring_fold_residual");
+}
+
+extern "C" int gm_ring_fold_residual_c(const double* F, const double* G, double* R, int64_t ld, int64_t nside,
+                                       int64_t lmax, void* stream) {
+    if (nside < 1 || lmax < 0) return fail(GM_E_ARG, "gm_ring_fold_residual_c: bad nside / lmax");
+    if (!F || !G || !R) return fail(GM_E_ARG, "gm_ring_fold_residual_c: null pointer");
+    const int64_t L = lmax + 1, nring = 4 * nside - 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_ring_fold_residual_c: ld < 2 lmax + 1");
+    hipLaunchKernelGGL(k_fold_s, dim3((unsigned)nring), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const double2*>(F), reinterpret_cast<const double2*>(G),
+                       reinterpret_cast<double2*>(R), ld, (int)nside, (int)L);
+    return check_launch("gm_ring_fold_residual_c");
+}

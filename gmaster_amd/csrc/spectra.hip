@@ -1,0 +1,116 @@
+// Power spectra and binning on gfx950, FP64: coupled pseudo-C_ell of healpy-packed alms
+// (HBM-bound segmented reduction over m), binned spectra and binned coupling blocks.
+#include "common.h"
+
+namespace gm {
+namespace {
+
+constexpr int kClMChunk = 32;  // orders per partial in the coupled-C_ell reduction
+
+// partial[mc][pair][l] = sum_{m in chunk mc, m <= l} c_m Re(a_lm conj b_lm)
+__global__ void k_cl_partial(const double2* __restrict__ a, int n1, const double2* __restrict__ b, int n2,
+                             int64_t stride, int L, double* __restrict__ part) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mc = blockIdx.y;
+    const int m0 = mc * kClMChunk;
+    if (l >= L || m0 > l) return;
+    const int mend = min(m0 + kClMChunk, l + 1);
+    const int lmax = L - 1;
+    const int np = n1 * n2;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int m = m0; m < mend; ++m) {
+        const int64_t ia = alm_index(l, m, lmax);
+        const double w = m ? 2.0 : 1.0;
+        for (int i = 0; i < n1; ++i) {
+            const double2 x = a[i * stride + ia];
+            for (int j = 0; j < n2; ++j) {
+                const double2 y = b[j * stride + ia];
+                acc[i * n2 + j] += w * (x.x * y.x + x.y * y.y);
+            }
+        }
+    }
+    for (int p = 0; p < np; ++p) part[((int64_t)mc * np + p) * L + l] = acc[p];
+}
+
+__global__ void k_cl_final(const double* __restrict__ part, int np, int L, double* __restrict__ cl) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (l >= L) return;
+    const int nmc = l / kClMChunk + 1;
+    double s = 0.0;
+    for (int mc = 0; mc < nmc; ++mc) s += part[((int64_t)mc * np + p) * L + l];
+    cl[(int64_t)p * L + l] = s / (2.0 * l + 1.0);
+}
+
+__global__ void k_bin_cell(const double* __restrict__ cl, int64_t ld, const int* __restrict__ ells,
+                           const int* __restrict__ bstart, int nb, const double* __restrict__ w,
+                           double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (b >= nb) return;
+    double s = 0.0;
+    for (int e = bstart[b]; e < bstart[b + 1]; ++e) s += w[e] * cl[c * ld + ells[e]];
+    out[(int64_t)c * nb + b] = s;
+}
+
+// out[b1][b2] = sum_{e1 in b1, e2 in b2} wout[e1] M[ells[e1]][ells[e2]] tin[e2]
+__global__ void k_bin_block(const double* __restrict__ M, int L, const int* __restrict__ ells,
+                            const int* __restrict__ bstart, int nb, const double* __restrict__ wout,
+                            const double* __restrict__ tin, double* __restrict__ out) {
+    const int b2 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b1 = blockIdx.y;
+    if (b2 >= nb) return;
+    double s = 0.0;
+    for (int e1 = bstart[b1]; e1 < bstart[b1 + 1]; ++e1) {
+        const double* row = M + (int64_t)ells[e1] * L;
+        double r = 0.0;
+        for (int e2 = bstart[b2]; e2 < bstart[b2 + 1]; ++e2) r += row[ells[e2]] * tin[e2];
+        s += wout[e1] * r;
+    }
+    out[(int64_t)b1 * nb + b2] = s;
+}
+
+}  // namespace
+}  // namespace gm
+
+using namespace gm;
+
+extern "C" int64_t gm_cl_work_size(int64_t lmax, int64_t npairs) {
+    const int64_t L = lmax + 1;
+    return ((L + kClMChunk - 1) / kClMChunk) * npairs * L;
+}
+
+extern "C" int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t n2, int64_t alm_stride,
+                               int64_t lmax, double* cl, double* work, void* stream) {
+    if (!alm1 || !alm2 || !cl || !work) return fail(GM_E_ARG, "gm_coupled_cell: null pointer");
+    if (lmax < 0) return fail(GM_E_ARG, "gm_coupled_cell: lmax < 0");
+    if (n1 < 1 || n1 > 2 || n2 < 1 || n2 > 2) return fail(GM_E_ARG, "gm_coupled_cell: 1 or 2 components per field");
+    const int64_t L = lmax + 1, nelem = L * (L + 1) / 2;
+    if ((n1 > 1 || n2 > 1) && alm_stride < nelem) return fail(GM_E_SHAPE, "gm_coupled_cell: alm stride too small");
+    const int nmc = (int)((L + kClMChunk - 1) / kClMChunk);
+    const int np = (int)(n1 * n2);
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_cl_partial, dim3((unsigned)((L + 255) / 256), nmc), dim3(256), 0, st,
+                       reinterpret_cast<const double2*>(alm1), (int)n1, reinterpret_cast<const double2*>(alm2),
+                       (int)n2, alm_stride, (int)L, work);
+    hipLaunchKernelGGL(k_cl_final, dim3((unsigned)((L + 255) / 256), np), dim3(256), 0, st, work, np, (int)L, cl);
+    return check_launch("gm_coupled_cell");
+}
+
+extern "C" int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells, const int32_t* band_start,
+                           int64_t nbands, const double* w, double* out, void* stream) {
+    if (!cl || !ells || !band_start || !w || !out) return fail(GM_E_ARG, "gm_bin_cell: null pointer");
+    if (ncl < 1 || nbands < 1) return fail(GM_E_ARG, "gm_bin_cell: empty input");
+    hipLaunchKernelGGL(k_bin_cell, dim3((unsigned)((nbands + 127) / 128), (unsigned)ncl), dim3(128), 0,
+                       (hipStream_t)stream, cl, ld, ells, band_start, (int)nbands, w, out);
+    return check_launch("gm_bin_cell");
+}
+
+extern "C" int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, const int32_t* band_start,
+                            int64_t nbands, const double* wout, const double* tin, double* out, void* stream) {
+    if (!M || !ells || !band_start || !wout || !tin || !out) return fail(GM_E_ARG, "gm_bin_block: null pointer");
+    if (lmax < 0 || nbands < 1) return fail(GM_E_ARG, "gm_bin_block: bad sizes");
+    hipLaunchKernelGGL(k_bin_block, dim3((unsigned)((nbands + 63) / 64), (unsigned)nbands), dim3(64), 0,
+                       (hipStream_t)stream, M, (int)(lmax + 1), ells, band_start, (int)nbands, wout, tin, out);
+    return check_launch("gm_bin_block");
+}

@@ -1,0 +1,171 @@
+/*
+ * gmaster_amd C-ABI: the drop-in boundary between the pymaster-compatible Python layer
+ * (gmaster_amd/*.py) and the hand-written gfx950 kernels (gmaster_amd/csrc/*.hip).
+ *
+ * Conventions shared by every entry point
+ *   - All pointers are device pointers (owned by the caller, e.g. torch tensors); no
+ *     entry point allocates device memory unless stated.  Complex arrays are interleaved
+ *     (re, im) doubles; "ld" arguments count complex elements per row.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls are asynchronous on it.
+ *   - Return 0 on success or a GM_E_* code; gm_last_error() gives the message (per thread).
+ *   - Every entry validates its sizes before launching, so the error paths are testable
+ *     without a GPU.
+ *
+ * Array layouts
+ *   ring spectrum, spin 0:  spec[map][ring][m],     m in [0, L),        ld >= L
+ *   ring spectrum, spin s:  spec[ring][m + L - 1],  m in (-L, L),       ld >= 2L - 1
+ *       "phased" spectra: spec[r][m] = (4 pi / npix) e^{-i m phi0_r} DFT_r[m mod nphi_r]
+ *   alm:                    healpy m-major packing, nelem = (lmax+1)(lmax+2)/2, complex
+ *   per-order tables:       order m owns l = m .. L-1+GM_TABLE_PAD at
+ *                           base(m) = m (L + GM_TABLE_PAD) - m (m - 1) / 2
+ *
+ * The reference (licongxu/GMaster) crosses this boundary through XLA FFI custom calls;
+ * each declaration cites the reference interface it replaces.
+ */
+#ifndef GMASTER_AMD_H
+#define GMASTER_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GM_TABLE_PAD 8
+
+enum {
+    GM_OK = 0,
+    GM_E_ARG = 1,         /* null pointer or invalid scalar argument */
+    GM_E_SHAPE = 2,       /* inconsistent sizes / leading dimensions */
+    GM_E_UNSUPPORTED = 3, /* valid request outside what this build supports */
+    GM_E_LAUNCH = 4       /* HIP launch / runtime error */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* gm_last_error(void);
+/* Version of this ABI (bumped on any signature change). */
+int gm_abi_version(void);
+
+/* Size in elements of one per-order table for band limit L = lmax + 1. */
+int64_t gm_table_size(int64_t lmax);
+
+/* ---------------------------------------------------------------- ring (azimuthal) stage */
+
+/* Forward ring FFT of nmaps real HEALPix maps (RING order) into phased spin-0 spectra,
+ * m in [0, lmax].  Replaces gmaster/_sht/rings.py:251-299 (_forward_ring_fft_positive)
+ * together with the weight / phase multiply of gmaster/_sht/healpix.py:72-94. */
+int gm_ring_fft_fwd_s0(const double* maps, int64_t nmaps, int64_t map_stride, int64_t nside,
+                       int64_t lmax, double* spec, int64_t ld, int64_t spec_map_stride, void* stream);
+
+/* Forward ring FFT of a spin field (Q, U) into the phased spectrum of Q + iU,
+ * m in (-L, L).  Replaces gmaster/_sht/rings.py:628-685 (_forward_ring_fft_full). */
+int gm_ring_fft_fwd_s(const double* q, const double* u, int64_t nside, int64_t lmax, double* spec,
+                      int64_t ld, void* stream);
+
+/* Inverse: maps[p] = sum_m F[r][m] e^{i m phi_p} (spin 0, Hermitian in m).
+ * Replaces gmaster/_sht/rings.py:477-524. */
+int gm_ring_fft_inv_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t spec_map_stride,
+                       int64_t nside, int64_t lmax, double* maps, int64_t map_stride, void* stream);
+
+/* Inverse for spin fields: Q + iU = sum_m F+[r][m] e^{i m phi_p}.
+ * Replaces gmaster/_sht/rings.py:702-771. */
+int gm_ring_fft_inv_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, double* q,
+                      double* u, void* stream);
+
+/* Jacobi residual in ring-Fourier space, no FFT:
+ *   R[r][k] = w nphi sum_{m = k mod nphi, |m| < L} e^{i (m-k) phi0} F[r][m] - G[r][k]
+ * (spin 0: F[-m] = conj F[m]).  In-place allowed (R == G).
+ * Replaces gm_ring_fold (gmaster/_native/cuda/march_v2.cu:426-470, sht/march_v2.py:614-625). */
+int gm_ring_fold_residual(const double* F, const double* G, double* R, int64_t nmaps, int64_t ld,
+                          int64_t map_stride, int64_t nside, int64_t lmax, void* stream);
+
+/* Spin-s version on (-L, L) spectra.  Replaces gm_ring_fold_c (march_v2.cu:474-493). */
+int gm_ring_fold_residual_c(const double* F, const double* G, double* R, int64_t ld,
+                            int64_t nside, int64_t lmax, void* stream);
+
+/* --------------------------------------------------------------- latitudinal (Legendre) */
+
+/* Recurrence tables for spin s (0 <= s <= lmax): mu-rescaled coefficients alpha, gamma
+ * (gamma may be NULL for s = 0) and output scale P, each gm_table_size(lmax) doubles;
+ * seed_mant / seed_exp (lmax+1 entries, host-computed) are passed to the kernels below.
+ * Replaces the window tables of gmaster/_sht/march_v2.py:388-450. */
+int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* gamma, double* P, void* stream);
+
+/* Spin-0 analysis of nmaps (1 or 2) phased spectra sharing one recurrence:
+ * partial[c][base(m) + l - m][map] = sum_{rings of chunk c} mu_lm(theta) (G_N +- G_S)
+ * Replaces gm_march_ana_s0 / gm_march_ana_s0_pair (march_v2.cu:155-285, :551-552). */
+int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t spec_map_stride,
+                  int64_t nside, int64_t lmax, const double* alpha, const double* seed_mant,
+                  const int32_t* seed_exp, const int32_t* nactive, double* partial,
+                  int64_t nchunks, void* stream);
+
+/* Sum chunk partials, scale by P and write alm (mode 0: alm = result; 1: alm -= result). */
+int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nchunks, const int32_t* nactive,
+                     const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
+                     void* stream);
+
+/* Synthesis coefficients: coef[base(m)+l-m][map] = alm[map][idx(l,m)] P_lm (zero padded). */
+int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_stride, const double* P,
+                   int64_t lmax, double* coef, void* stream);
+
+/* Spin-0 synthesis: spec[map][r][m] = sum_l coef * mu_lm(theta_r).
+ * Replaces gm_march_syn_s0 / _pair (march_v2.cu:291-407, :554-555). */
+int gm_lat_syn_s0(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
+                  const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+                  double* spec, int64_t ld, int64_t spec_map_stride, void* stream);
+
+/* Spin-s analysis over all rings (direct + mirror channel per order m >= 0).
+ * Replaces gm_march_ana_s2 (march_v2.cu:553). */
+int gm_lat_ana_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
+                 const double* alpha, const double* gamma, const double* seed_mant,
+                 const int32_t* seed_exp, const int32_t* nactive, double* partial, int64_t nchunks,
+                 void* stream);
+
+/* Finish spin s: sum partials, scale, combine +-m into E/B (healpy packing, 2 x nelem).
+ * Replaces gmaster/_sht/healpix.py:117-143 and :812-814. */
+int gm_lat_finish_s(const double* partial, int64_t nchunks, const int32_t* nactive, const double* P,
+                    int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream);
+
+/* Synthesis coefficients for spin s from E/B (gmaster/_sht/healpix.py:599-611). */
+int gm_lat_prep_s(const double* alm_e, const double* alm_b, const double* P, int64_t lmax,
+                  int64_t spin, double* coef, void* stream);
+
+/* Spin-s synthesis into spec[r][m + L - 1].  Replaces gm_march_syn_s2 (march_v2.cu:556). */
+int gm_lat_syn_s(const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
+                 const double* gamma, const double* seed_mant, const int32_t* seed_exp,
+                 const int32_t* nactive, double* spec, int64_t ld, void* stream);
+
+/* Number of ring chunks the analysis kernels split the rings of one order into. */
+int64_t gm_lat_nchunks(int64_t nside, int64_t spin);
+
+/* ---------------------------------------------------------------- power spectra, MCM */
+
+/* Coupled pseudo-C_ell of every (a, b) pair: cl[a*n2+b][l] = sum_m c_m Re(a conj b)/(2l+1).
+ * work: >= gm_cl_work_size(lmax, n1*n2) doubles.  Replaces gmaster/workspaces.py:86-95. */
+int64_t gm_cl_work_size(int64_t lmax, int64_t npairs);
+int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t n2, int64_t alm_stride,
+                    int64_t lmax, double* cl, double* work, void* stream);
+
+/* Mode-coupling blocks 0-0, 0-2, 2-2 even / odd in one pass over the Wigner-3j rows:
+ *   M[l1][l2] = (2 l2 + 1) sum_l3 W'_l3 (l1 l2 l3; ...)(l1 l2 l3; ...),  W'_l = (2l+1) W_l / (4 pi)
+ * window: lmax+1 doubles (the mask pseudo-C_ell, zero above lmax); g: g(p) for p < lmax+2.
+ * Any of m00, m02, m22p, m22m may be NULL.  Replaces gm_coupling_tt
+ * (gmaster/_native/cuda/coupling_tt.cu:18-81) and the quadrature of gmaster/workspaces.py:682-732. */
+int gm_coupling(const double* window, const double* g, int64_t lmax, double* m00, double* m02,
+                double* m22p, double* m22m, void* stream);
+
+/* Binned block: out[b1][b2] = sum_{l1 in b1, l2 in b2} wout[l1] M[l1][l2] tin[l2]
+ * (CSR bands: ells sorted by band, band_start has nbands+1 entries).
+ * Replaces gmaster/workspaces.py:292-310 (_BlockMCM.banded). */
+int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, const int32_t* band_start,
+                 int64_t nbands, const double* wout, const double* tin, double* out, void* stream);
+
+/* Binned spectra: out[c][b] = sum_{l in b} w[l] cl[c][l].  Replaces gmaster/bins.py:15-20. */
+int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
+                const int32_t* band_start, int64_t nbands, const double* w, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,108 @@
+"""Mode-coupling matrices for spin pairs 0-0, 0-2 and 2-2 (oracle, FP64).
+
+The reference's general coupling matrix (``gmaster/workspaces.py:1069-1077``) is
+
+    M[l1, l2] = (2 l2 + 1)/(4 pi) sum_l3 (2 l3 + 1) W_l3 P(J) (l1 l2 l3; n1 -s1 s1-n1)(l1 l2 l3; n2 -s2 s2-n2)
+
+with ``W`` the mask pseudo-C_ell padded with zeros above lmax_mask
+(``gmaster/workspaces.py:1778-1781``).  The blocks used by MASTER are
+``M00`` (0,0,0,0), ``M02`` = sum over both parities of (0,2,0,2) (the TE block,
+``gmaster/workspaces.py:912-918``) and ``M22+`` / ``M22-`` = even / odd parity of
+(2,2,2,2) (``:919-922``).
+
+The oracle evaluates them by Gauss-Legendre quadrature of Wigner-d products,
+``int d^l1_{a a'} d^l2_{b b'} d^l3_{c c'} dx = 2 (3j)(3j)``, with the mask correlation
+function ``xi(x) = sum (2 l3 + 1) W_l3 P_l3(x)``: the same identity the reference's
+quadrature route uses (``gmaster/workspaces.py:682-732``), and an algorithm independent
+of the l3 recursion in the HIP coupling kernel.  Pinned against exact Racah 3j sums
+(``oracle.wigner3j.threej_exact``) at small lmax and against the full-sky invariant
+(W = 4 pi delta_{l0} gives M = I, M22- = 0) in ``tests/test_oracle_coupling.py``.
+"""
+
+import math
+
+import numpy as np
+from scipy.special import roots_legendre
+
+from .legendre import lambda_rows
+
+
+def _d_rows(L, m, s, x):
+    """``d^l_{m,-s}(x)`` rows (unnormalised) for l < L at nodes x."""
+    lam = lambda_rows(L, m, s, x, np.sqrt((1 - x) / 2), np.sqrt((1 + x) / 2))
+    ell = np.arange(L)
+    return lam / np.sqrt((2 * ell + 1) / (4 * math.pi))[:, None]
+
+
+def coupling_blocks(window, lmax):
+    """``(M00, M02, M22p, M22m)`` each ``(lmax+1, lmax+1)`` from ``window`` (>= lmax+1 entries,
+    only the first lmax+1 are used, matching the zero padding above lmax_mask = lmax)."""
+    L = lmax + 1
+    W = np.zeros(L)
+    W[: min(L, len(window))] = np.asarray(window, dtype=np.float64)[:L]
+    n = (3 * lmax) // 2 + 2
+    x, w = roots_legendre(n)
+    P = _d_rows(L, 0, 0, x)  # d_00 = P_l
+    xi = ((2 * np.arange(L) + 1) * W) @ P
+    d20 = _d_rows(L, 2, 0, x)  # d_{2,0} = d_{0,2}
+    d0m2 = _d_rows(L, 0, 2, x)  # d_{0,-2}
+    d2m2 = _d_rows(L, 2, 2, x)  # d_{2,-2}  (= d_{-2,2})
+    # d_{2,2}(x) = d_{-2,-2}(x) = (-1)^l d_{2,-2}(-x)
+    d22 = _d_rows(L, 2, 2, -x) * ((-1.0) ** np.arange(L))[:, None]
+    wx = w * xi
+    col = (2 * np.arange(L) + 1)[None, :] / (8 * math.pi)
+    M00 = (P * wx) @ P.T * col
+    M02 = (d20 * wx) @ d0m2.T * col
+    Mall = (d22 * wx) @ d22.T * col
+    Msig = (d2m2 * wx) @ d2m2.T * col
+    M22p = 0.5 * (Mall + Msig)
+    M22m = 0.5 * (Mall - Msig)
+    return M00, M02, M22p, M22m
+
+
+def coupling_blocks_exact(window, lmax):
+    """The same four blocks from exact Racah 3j symbols (tiny lmax only)."""
+    from .wigner3j import threej_exact
+
+    L = lmax + 1
+    W = np.zeros(L)
+    W[: min(L, len(window))] = np.asarray(window, dtype=np.float64)[:L]
+    out = np.zeros((4, L, L))
+    for l1 in range(L):
+        for l2 in range(L):
+            acc = np.zeros(4)
+            for l3 in range(abs(l1 - l2), min(l1 + l2, lmax) + 1):
+                w0 = threej_exact(l1, l2, l3, 0, 0, 0)
+                w2 = threej_exact(l1, l2, l3, 2, -2, 0)
+                f = (2 * l3 + 1) * W[l3]
+                acc[0] += f * w0 * w0
+                acc[1] += f * w0 * w2
+                if (l1 + l2 + l3) % 2 == 0:
+                    acc[2] += f * w2 * w2
+                else:
+                    acc[3] += f * w2 * w2
+            out[:, l1, l2] = acc * (2 * l2 + 1) / (4 * math.pi)
+    return tuple(out)
+
+
+def assemble_mcm(blocks, spin1, spin2):
+    """Dense MCM in ``l * ncls + c`` ordering (``gmaster/workspaces.py:1851-1903``, ``:313-342``)."""
+    M00, M02, M22p, M22m = blocks
+    L = M00.shape[0]
+    if spin1 == 0 and spin2 == 0:
+        return M00.copy(), 1
+    if (spin1 == 0) != (spin2 == 0):
+        ncls = 2
+        out = np.zeros((L, ncls, L, ncls))
+        sign = (-1) ** (spin1 + spin2)
+        for c in range(2):
+            out[:, c, :, c] += sign * M02
+        return out.reshape(L * ncls, L * ncls), ncls
+    ncls = 4
+    sign = (-1) ** (spin1 + spin2)
+    out = np.zeros((L, ncls, L, ncls))
+    for c in range(4):
+        out[:, c, :, c] += sign * M22p
+    for c in range(4):
+        out[:, c, :, 3 - c] += (-sign if c in (1, 2) else sign) * M22m
+    return out.reshape(L * ncls, L * ncls), ncls

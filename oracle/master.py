@@ -1,0 +1,81 @@
+"""Coupled C_ell, binning and decoupling (oracle, FP64).
+
+Restates ``gmaster/workspaces.py:86-95`` (coupled C_ell), ``gmaster/bins.py:30-37,
+65-101, 228-250`` (linear bands, weights, ``bin_cell``), ``gmaster/workspaces.py:
+1452-1470, 1978-2003`` (binning operators, binned MCM) and ``:2106-2147``
+(``decouple_cell``).  Pinned by the direct definitions (explicit loops over m and
+bands in ``tests/test_oracle_master.py``) and, end to end, by the full-sky identity
+(decoupled bandpowers == binned input pseudo-C_ell when the mask is 1).
+"""
+
+import numpy as np
+
+from .coupling import assemble_mcm, coupling_blocks
+from .geometry import AlmInfo
+
+
+def coupled_cell(alm1, alm2, lmax):
+    """``C_l^{ab} = sum_m c_m Re(a_lm conj b_lm) / (2l + 1)``, c_0 = 1, c_{m>0} = 2; rows a*n2 + b."""
+    ainfo = AlmInfo(lmax)
+    alm1 = np.atleast_2d(alm1)
+    alm2 = np.atleast_2d(alm2)
+    wt = np.where(ainfo.m == 0, 1.0, 2.0)
+    out = []
+    for a in alm1:
+        for b in alm2:
+            prod = (a * np.conj(b)).real * wt
+            out.append(np.bincount(ainfo.ell, weights=prod, minlength=lmax + 1) / (2 * np.arange(lmax + 1) + 1))
+    return np.array(out)
+
+
+def linear_bins(lmax, nlb):
+    """``(ells, bpws, weights)``: bands of ``nlb`` from l = 2, incomplete last band dropped."""
+    ells = np.arange(lmax + 1)
+    bpws = (ells - 2) // nlb
+    bpws[:2] = -1
+    if np.sum(bpws == bpws[-1]) != nlb:
+        bpws[bpws == bpws[-1]] = -1
+    keep = bpws >= 0
+    ells, bpws = ells[keep], bpws[keep]
+    nb = bpws.max() + 1
+    w = np.ones(len(ells))
+    w /= np.bincount(bpws, weights=w, minlength=nb)[bpws]
+    return ells, bpws, w
+
+
+def binning_operators(lmax, ells, bpws, weights, f_ell=None):
+    nb = bpws.max() + 1
+    f = np.ones(len(ells)) if f_ell is None else f_ell
+    out = np.zeros((nb, lmax + 1))
+    out[bpws, ells] = weights * f
+    th = np.zeros((lmax + 1, nb))
+    th[ells, bpws] = 1.0 / f
+    return out, th
+
+
+def bin_cell(cls, lmax, ells, bpws, weights):
+    nb = bpws.max() + 1
+    cls = np.atleast_2d(cls)
+    return np.array([np.bincount(bpws, weights=c[ells] * weights, minlength=nb) for c in cls])
+
+
+def binned_mcm(mcm, ncls, out_op, th_op):
+    eye = np.eye(ncls)
+    W = np.kron(out_op, eye)
+    T = np.kron(th_op, eye)
+    return W @ mcm @ T
+
+
+def decouple(mcm_binned, cl_coupled, lmax, ells, bpws, weights, ncls):
+    nb = bpws.max() + 1
+    binned = bin_cell(cl_coupled, lmax, ells, bpws, weights).T.reshape(-1)
+    return np.linalg.solve(mcm_binned, binned).reshape((nb, ncls)).T
+
+
+def workspace(pcl_mask, lmax, spin1, spin2, nlb):
+    """Binned MCM for a spin pair from the mask pseudo-C_ell (window zero above lmax)."""
+    blocks = coupling_blocks(pcl_mask, lmax)
+    mcm, ncls = assemble_mcm(blocks, spin1, spin2)
+    ells, bpws, w = linear_bins(lmax, nlb)
+    out_op, th_op = binning_operators(lmax, ells, bpws, w)
+    return mcm, binned_mcm(mcm, ncls, out_op, th_op), ncls, (ells, bpws, w)

@@ -1,0 +1,71 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the header
+declares, and rejects bad arguments before launching anything."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from gmaster_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "gmaster_amd.h")).read()
+    decl = r"^\s*(?:const\s+char\s*\*|int64_t|int)\s+(gm_[a-z0-9_]+)\s*\("
+    return sorted(set(re.findall(decl, text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _header_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    # and the Python binding declares every one of them
+    assert set(names) <= set(_lib.SIGNATURES)
+
+
+def test_abi_version():
+    assert _lib.query("gm_abi_version") == 1
+
+
+def test_table_size_matches_layout():
+    for lmax in (0, 5, 191, 12287):
+        L = lmax + 1
+        expect = L * (L + 8) - L * (L - 1) // 2
+        assert _lib.query("gm_table_size", lmax) == expect
+
+
+def test_nchunks():
+    assert _lib.query("gm_lat_nchunks", 64, 0) == 1
+    assert _lib.query("gm_lat_nchunks", 4096, 0) == 4
+    assert _lib.query("gm_lat_nchunks", 4096, 2) == 8
+
+
+@pytest.mark.parametrize(
+    "name,args,code",
+    [
+        ("gm_ring_fft_fwd_s0", (None, 1, 12, 3, 2, None, 3, 0, None), 1),  # nside not a power of 2
+        ("gm_ring_fft_fwd_s0", (1, 1, 768, 8, 10, 1, 5, 0, None), 2),     # ld < lmax + 1
+        ("gm_ring_fft_fwd_s0", (1, 1, 0, 16384, 10, 1, 11, 0, None), 3),  # nside above the FFT limit
+        ("gm_ring_fft_fwd_s", (1, 1, 8, 10, 1, 20, None), 2),            # ld < 2 lmax + 1
+        ("gm_lat_ana_s0", (1, 3, 11, 0, 8, 10, 1, 1, 1, 1, 1, 1, None), 1),  # nmaps = 3
+        ("gm_lat_ana_s0", (1, 1, 11, 0, 8, 10, 1, 1, 1, 1, 1, 7, None), 2),  # wrong chunk count
+        ("gm_lat_ana_s", (1, 21, 8, 10, 0, 1, 1, 1, 1, 1, 1, 1, None), 1),   # spin 0 on the spin-s entry
+        ("gm_lat_tables", (10, 11, 1, 1, 1, None), 1),                      # spin > lmax
+        ("gm_coupling", (None, 1, 10, 1, None, None, None, None), 1),       # null window
+        ("gm_coupled_cell", (1, 3, 1, 1, 66, 10, 1, 1, None), 1),           # 3 components
+    ],
+)
+def test_argument_validation(name, args, code):
+    conv = [ctypes.c_void_p(a) if (a is not None and isinstance(a, int) and t is ctypes.c_void_p) else a
+            for a, t in zip(args, _lib.SIGNATURES[name])]
+    lib = _lib.load()
+    got = getattr(lib, name)(*conv)
+    assert got == code, lib.gm_last_error().decode()
+    assert lib.gm_last_error().decode()
+    with pytest.raises(_lib.GmError):
+        _lib.call(name, *conv)
