@@ -1,3 +1,44 @@
-"""gmaster_amd: a pymaster-compatible curved-sky MASTER engine on hand-written gfx950 kernels."""
+"""gmaster_amd: a pymaster-compatible curved-sky MASTER engine on hand-written gfx950 kernels.
+
+``import gmaster_amd as nmt`` (or ``import gmaster as nmt``) exposes the NaMaster API of
+the curved-sky HEALPix path: ``NmtField``, ``NmtBin``, ``NmtWorkspace.from_fields``,
+``compute_coupled_cell``, ``decouple_cell``, ``map2alm`` / ``alm2map`` of any spin.
+Every transform, coupling matrix and spectrum runs in the HIP library
+``libgmaster_amd.so`` (built by ``python -m gmaster_amd.build``); there is no CPU path.
+"""
 
 from . import _lib, sht  # noqa: F401
+from ._array import GmArray
+from ._config import get_default_params, nmt_params, set_n_iter_default, set_sht_calculator, set_tol_pinv_default
+from .bins import NmtBin
+from .field import NmtField
+from .utils import (
+    NmtAlmInfo,
+    NmtMapInfo,
+    alm2map,
+    map2alm,
+    map2alm_pair,
+    mask_apodization,
+    synfast_spherical,
+)
+from .workspaces import NmtWorkspace, compute_coupled_cell
+
+__all__ = [
+    "GmArray",
+    "NmtAlmInfo",
+    "NmtBin",
+    "NmtField",
+    "NmtMapInfo",
+    "NmtWorkspace",
+    "alm2map",
+    "compute_coupled_cell",
+    "get_default_params",
+    "map2alm",
+    "map2alm_pair",
+    "mask_apodization",
+    "nmt_params",
+    "set_n_iter_default",
+    "set_sht_calculator",
+    "set_tol_pinv_default",
+    "synfast_spherical",
+]

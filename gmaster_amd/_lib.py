@@ -34,7 +34,7 @@ SIGNATURES = {
     "gm_lat_prep_s": [_p, _p, _p, _i64, _i64, _p, _p],
     "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_coupled_cell": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p],
-    "gm_coupling": [_p, _p, _i64, _p, _p, _p, _p, _p],
+    "gm_coupling": [_p, _i64, _p, _i64, _p, _p, _p, _p, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
     "gm_table_size": [_i64],
@@ -80,9 +80,28 @@ def load():
     return lib
 
 
+# Optional per-entry-point timing with events on the launching stream (used by bench.py):
+# TIMERS[name] collects (start, end) torch.cuda.Event pairs while enabled.
+TIMERS = None
+
+
+def enable_timers(on=True):
+    global TIMERS
+    TIMERS = {} if on else None
+    return TIMERS
+
+
 def call(name, *args):
     lib = load()
-    code = getattr(lib, name)(*args)
+    if TIMERS is not None and not name.startswith(("gm_table", "gm_last", "gm_abi")):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        code = getattr(lib, name)(*args)
+        end.record()
+        TIMERS.setdefault(name, []).append((start, end))
+    else:
+        code = getattr(lib, name)(*args)
     if code != 0:
         raise GmError(name, code, lib.gm_last_error().decode())
     return code

@@ -19,17 +19,16 @@
 namespace gm {
 namespace {
 
-__global__ void k_coupling(const double* __restrict__ W, const double* __restrict__ g, int L,
+__global__ void k_coupling(const double* __restrict__ W, int lmax_w, const double* __restrict__ g, int L,
                            double* __restrict__ m00, double* __restrict__ m02, double* __restrict__ m22p,
                            double* __restrict__ m22m) {
     const int l1 = blockIdx.y;
     const int l2 = l1 + blockIdx.x * blockDim.x + threadIdx.x;
     if (l2 >= L) return;
-    const int lmax = L - 1;
     const int d = l2 - l1;
     const double S = (double)(l1 + l2 + 1);
     const double dd = (double)d;
-    const int l3max = min(l1 + l2, lmax);
+    const int l3max = min(l1 + l2, lmax_w);
 
     double w0 = sqrt(g[l1] * g[d] / ((2.0 * l2 + 1.0) * g[l2]));
     if (l2 & 1) w0 = -w0;
@@ -91,14 +90,16 @@ __global__ void k_coupling(const double* __restrict__ W, const double* __restric
 
 using namespace gm;
 
-extern "C" int gm_coupling(const double* window, const double* g, int64_t lmax, double* m00, double* m02,
+extern "C" int gm_coupling(const double* window, int64_t lmax_window, const double* g, int64_t lmax, double* m00,
+                           double* m02,
                            double* m22p, double* m22m, void* stream) {
     if (!window || !g) return fail(GM_E_ARG, "gm_coupling: null window / g table");
     if (lmax < 0 || lmax > 65535) return fail(GM_E_ARG, "gm_coupling: bad lmax %lld", (long long)lmax);
     if (!m00 && !m02 && !m22p && !m22m) return fail(GM_E_ARG, "gm_coupling: no output requested");
+    if (lmax_window < 0 || lmax_window > 2 * lmax) return fail(GM_E_ARG, "gm_coupling: need 0 <= lmax_window <= 2 lmax");
     const int L = (int)lmax + 1;
     const int bx = 128;
     hipLaunchKernelGGL(k_coupling, dim3((unsigned)((L + bx - 1) / bx), (unsigned)L), dim3(bx), 0,
-                       (hipStream_t)stream, window, g, L, m00, m02, m22p, m22m);
+                       (hipStream_t)stream, window, (int)lmax_window, g, L, m00, m02, m22p, m22m);
     return check_launch("gm_coupling");
 }

@@ -1,0 +1,142 @@
+"""Curved-sky fields (``pymaster.NmtField``), restating ``gmaster/field.py:78-376``.
+
+A field masks its maps and analyses them (``n_iter`` Jacobi steps).  A spin-0 field
+whose mask has the same band limit and ``n_iter`` analyses the masked map and the mask
+in one shared latitudinal pass (``gmaster/field.py:273-289``); other fields compute
+their mask alms lazily (``:348-376``).  Contaminant deprojection, purification and
+anisotropic masks are outside this engine and raise ``NotImplementedError``.
+"""
+
+import torch
+
+from . import sht
+from ._array import device_tensor, wrap
+from ._config import nmt_params
+from .utils import NmtAlmInfo, NmtMapInfo
+
+# Mask alms shared by fields built on the same, unmodified mask tensor (keyed on its storage,
+# in-place version counter, band limit and n_iter), so that every workspace of one mask
+# sees bit-identical mask spectra and shares one coupling computation.
+_MASK_ALMS = {}
+
+
+def _mask_key(mask, lmax_mask, n_iter_mask):
+    return (mask.data_ptr(), mask._version, mask.numel(), str(mask.device), int(lmax_mask), int(n_iter_mask))
+
+
+def clear_mask_cache():
+    _MASK_ALMS.clear()
+
+
+def _remember(key, alm):
+    if len(_MASK_ALMS) >= 4:
+        _MASK_ALMS.clear()
+    _MASK_ALMS[key] = alm
+
+
+class NmtField:
+    def __init__(self, mask, maps, *, spin=None, templates=None, beam=None, purify_e=False, purify_b=False,
+                 n_iter=None, n_iter_mask=None, tol_pinv=None, wcs=None, lmax=None, lmax_mask=None,
+                 masked_on_input=False, lite=False, mask_22=None, mask_12=None):
+        if templates is not None:
+            raise NotImplementedError("contaminant deprojection (templates) is not supported by this engine")
+        if purify_e or purify_b:
+            raise NotImplementedError("E/B purification is not supported by this engine")
+        if mask_22 is not None or mask_12 is not None:
+            raise NotImplementedError("anisotropic masks are not supported by this engine")
+        self.lite = bool(lite)
+        self.n_iter = nmt_params.n_iter_default if n_iter is None else int(n_iter)
+        self.n_iter_mask = nmt_params.n_iter_mask_default if n_iter_mask is None else int(n_iter_mask)
+        self.pure_e = self.pure_b = False
+        self.is_catalog = False
+        self.anisotropic_mask = False
+        self.alm = self.alm_mask = None
+        self.maps = None
+        self.n_temp = 0
+        self.Nw = 0.0
+        self.Nf = 0.0
+
+        mask_t = device_tensor(mask)
+        if mask_t.ndim != 1:
+            raise ValueError("Mask must be a one-dimensional HEALPix map")
+        self.minfo = NmtMapInfo(wcs, mask_t.shape)
+        self.mask = mask_t
+        lmax = self.minfo.get_lmax() if lmax is None else int(lmax)
+        lmax_mask = self.minfo.get_lmax() if lmax_mask is None else int(lmax_mask)
+        if lmax <= 0 or lmax_mask <= 0:
+            raise ValueError("`lmax` and `lmax_mask` must be positive")
+        self.ainfo = NmtAlmInfo(lmax)
+        self.ainfo_mask = NmtAlmInfo(lmax_mask)
+        dev = mask_t.device
+        if beam is None:
+            self.beam = torch.ones(lmax + 1, dtype=torch.float64, device=dev)
+        else:
+            b = device_tensor(beam, device=dev)
+            if b.ndim != 1 or len(b) <= lmax:
+                raise ValueError(f"Input beam must have at least {lmax + 1} elements given the input map resolution")
+            self.beam = b[: lmax + 1]
+
+        if maps is None:
+            if spin is None:
+                raise ValueError("Please supply field spin")
+            self.spin = int(spin)
+            self.nmaps = 1 if self.spin == 0 else 2
+            return
+
+        maps_t = device_tensor(maps, device=dev)
+        if maps_t.ndim == 1:
+            maps_t = maps_t[None]
+        if maps_t.ndim != 2 or len(maps_t) not in (1, 2):
+            raise ValueError("Must supply 1 or 2 maps per field")
+        if maps_t.shape[1] != self.minfo.npix:
+            raise ValueError("Incompatible map!")
+        if spin is None:
+            spin = 0 if len(maps_t) == 1 else 2
+        if (spin == 0) != (len(maps_t) == 1):
+            raise ValueError("Spin-zero fields are associated with a single map")
+        self.spin = int(spin)
+        self.nmaps = 1 if self.spin == 0 else 2
+
+        masked = maps_t if masked_on_input else maps_t * self.mask[None, :]
+        key = _mask_key(self.mask, lmax_mask, self.n_iter_mask)
+        self.alm_mask = _MASK_ALMS.get(key)
+        if (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
+                and self.n_iter == self.n_iter_mask and self.alm_mask is None):
+            pair = sht.map2alm_s0(torch.cat([masked, self.mask[None, :]]), lmax, n_iter=self.n_iter)
+            self.alm = pair[0:1]
+            self.alm_mask = pair[1]
+            _remember(key, self.alm_mask)
+        else:
+            self.alm = sht.map2alm(masked, self.spin, lmax, n_iter=self.n_iter)
+        if not self.lite:
+            self.maps = masked
+
+    def is_compatible(self, other, strict=True):
+        if strict and self.minfo != other.minfo:
+            return False
+        return self.ainfo_mask == other.ainfo_mask and self.ainfo == other.ainfo
+
+    def get_mask(self):
+        return wrap(self.mask)
+
+    def get_maps(self):
+        if self.maps is None:
+            raise ValueError("Input maps unavailable for this field")
+        return wrap(self.maps)
+
+    def get_alms(self):
+        if self.alm is None:
+            raise ValueError("Input maps unavailable for this field")
+        return wrap(self.alm)
+
+    def get_mask_alms(self):
+        if self.alm_mask is None:
+            key = _mask_key(self.mask, self.ainfo_mask.lmax, self.n_iter_mask)
+            alm = _MASK_ALMS.get(key)
+            if alm is None:
+                alm = sht.map2alm_s0(self.mask[None, :], self.ainfo_mask.lmax, n_iter=self.n_iter_mask)[0]
+                _remember(key, alm)
+            if self.lite:
+                return wrap(alm)
+            self.alm_mask = alm
+        return wrap(self.alm_mask)

@@ -1,0 +1,253 @@
+"""MASTER workspaces (``pymaster.NmtWorkspace``) and coupled spectra.
+
+Restates ``gmaster/workspaces.py``:
+
+* ``compute_coupled_cell`` (``:86-125``) -- kernel ``gm_coupled_cell``;
+* the mode-coupling blocks 00, 02 (TE), 22+ / 22- (``:352-517, 878-924``) -- one pass of
+  the Wigner-3j kernel ``gm_coupling`` produces all four, so workspaces built from the
+  same mask window share it (cached on the window's content);
+* the slot / sign layout of the blocks (``:1851-1903``), binning with weights, f_ell and
+  beams (``:1452-1470, :292-310``) -- kernel ``gm_bin_block``;
+* ``decouple_cell`` (``:2106-2147``): binned spectra (``gm_bin_cell``) and an FP64 LU
+  solve of the binned MCM (``torch.linalg.solve``, LAPACK-style getrf/getrs on the GPU).
+"""
+
+import hashlib
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._array import device_tensor, wrap
+
+_COUPLING_CACHE = {}
+_G_CACHE = {}
+
+
+def _g_table(n, device):
+    """``g(p) = Gamma(p + 1/2) / (sqrt(pi) Gamma(p + 1))`` for p < n, long-double recursion."""
+    key = (n, str(device))
+    t = _G_CACHE.get(key)
+    if t is None:
+        p = np.arange(1, n, dtype=np.longdouble)
+        g = np.concatenate([[np.longdouble(1)], np.cumprod((2 * p - 1) / (2 * p))])
+        t = torch.as_tensor(g.astype(np.float64), device=device)
+        _G_CACHE[key] = t
+    return t
+
+
+def coupling_blocks(window, lmax):
+    """``{"00", "02", "22p", "22m"}`` coupling blocks ``(lmax+1, lmax+1)`` for a mask window.
+
+    ``window`` is the mask pseudo-C_ell (entries above 2 lmax cannot couple).  Results are
+    cached on (lmax, window bytes) so the 00, 02 and 22 workspaces of one mask share a pass.
+    """
+    w = device_tensor(window)[: 2 * lmax + 1].contiguous()
+    key = (int(lmax), str(w.device), hashlib.sha1(w.cpu().numpy().tobytes()).hexdigest())
+    hit = _COUPLING_CACHE.get(key)
+    if hit is not None:
+        return hit
+    L = lmax + 1
+    dev = w.device
+    blocks = {k: torch.empty((L, L), dtype=torch.float64, device=dev) for k in ("00", "02", "22p", "22m")}
+    g = _g_table(L + 1, dev)
+    _lib.call("gm_coupling", _lib.ptr(w), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(blocks["00"]),
+              _lib.ptr(blocks["02"]), _lib.ptr(blocks["22p"]), _lib.ptr(blocks["22m"]), _lib.stream_ptr(dev))
+    if len(_COUPLING_CACHE) >= 2:
+        _COUPLING_CACHE.clear()
+    _COUPLING_CACHE[key] = blocks
+    return blocks
+
+
+def clear_coupling_cache():
+    _COUPLING_CACHE.clear()
+
+
+def compute_coupled_cell(f1, f2):
+    """``(n1 n2, lmax + 1)`` coupled pseudo-spectra of two fields (rows ``a * n2 + b``)."""
+    if not f1.is_compatible(f2, strict=False):
+        raise ValueError("You're trying to correlate incompatible fields")
+    return wrap(_coupled_cell(f1.get_alms(), f2.get_alms(), min(f1.ainfo.lmax, f2.ainfo.lmax)))
+
+
+def _coupled_cell(alm1, alm2, lmax):
+    a1 = device_tensor(alm1, dtype=torch.complex128)
+    a2 = device_tensor(alm2, dtype=torch.complex128, device=a1.device)
+    if a1.ndim == 1:
+        a1 = a1[None]
+    if a2.ndim == 1:
+        a2 = a2[None]
+    n1, n2 = a1.shape[0], a2.shape[0]
+    if a1.shape[1] != a2.shape[1]:
+        raise ValueError("alms of different sizes")
+    L = lmax + 1
+    cl = torch.empty((n1 * n2, L), dtype=torch.float64, device=a1.device)
+    work = torch.empty(int(_lib.query("gm_cl_work_size", lmax, n1 * n2)), dtype=torch.float64, device=a1.device)
+    _lib.call("gm_coupled_cell", _lib.ptr(a1), n1, _lib.ptr(a2), n2, a1.shape[1], lmax, _lib.ptr(cl), _lib.ptr(work),
+              _lib.stream_ptr(a1.device))
+    return cl
+
+
+def _layout(ncls, spin1, spin2):
+    """``[(block name, (row channel, column channel), sign)]`` (``gmaster/workspaces.py:1851-1903``)."""
+    if ncls == 1:
+        return [("00", (0, 0), 1.0)]
+    sign = float((-1) ** (spin1 + spin2))
+    if ncls == 2:
+        return [("02", (0, 0), sign), ("02", (1, 1), sign)]
+    out = [("22p", (c, c), sign) for c in range(4)]
+    out += [("22m", (c, 3 - c), -sign if c in (1, 2) else sign) for c in range(4)]
+    return out
+
+
+class NmtWorkspace:
+    """Mode-coupling matrix, its binned form and the bandpower operators of two fields."""
+
+    def __init__(self, fl1=None, fl2=None, bins=None, is_teb=False, l_toeplitz=-1, l_exact=-1, dl_band=-1,
+                 fname=None, normalization="MASTER"):
+        self.blocks = None
+        self.mcm_binned = None
+        self._bpws = None
+        if fname is not None:
+            raise NotImplementedError("workspace FITS I/O is not provided by this engine")
+        if fl1 is not None or fl2 is not None or bins is not None:
+            self.compute_coupling_matrix(fl1, fl2, bins, is_teb=is_teb, l_toeplitz=l_toeplitz, l_exact=l_exact,
+                                         dl_band=dl_band, normalization=normalization)
+
+    @classmethod
+    def from_fields(cls, fl1, fl2, bins, **kwargs):
+        return cls(fl1, fl2, bins, **kwargs)
+
+    def compute_coupling_matrix(self, fl1, fl2, bins, is_teb=False, l_toeplitz=-1, l_exact=-1, dl_band=-1,
+                                normalization="MASTER"):
+        if not fl1.is_compatible(fl2, strict=False):
+            raise ValueError("Fields have incompatible pixelizations")
+        if fl1.ainfo.lmax != bins.lmax:
+            raise ValueError(f"Maximum multipoles in bins ({bins.lmax}) and fields ({fl1.ainfo.lmax}) are not the same")
+        if is_teb:
+            raise NotImplementedError("joint TEB workspaces are not supported by this engine")
+        if l_toeplitz > 0:
+            raise NotImplementedError("the Toeplitz approximation is not supported by this engine")
+        if normalization not in ("MASTER", "FKP"):
+            raise ValueError(f"Unknown normalization type {normalization}. Allowed options are 'MASTER' and 'FKP'")
+        for f in (fl1, fl2):
+            if f.spin not in (0, 2):
+                raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
+        self.spin1, self.spin2 = fl1.spin, fl2.spin
+        self.nmaps1 = 1 if self.spin1 == 0 else 2
+        self.nmaps2 = 1 if self.spin2 == 0 else 2
+        self.ncls = self.nmaps1 * self.nmaps2
+        self.lmax = fl1.ainfo.lmax
+        self.lmax_mask = fl1.ainfo_mask.lmax
+        self.beam1 = fl1.beam
+        self.beam2 = fl2.beam
+        self.bins = bins
+        self.nbands = bins.get_n_bands()
+        self.normalization = normalization
+        self.norm_type = int(normalization == "FKP")
+        am1 = fl1.get_mask_alms()
+        am2 = am1 if fl2 is fl1 else fl2.get_mask_alms()
+        self.pcl_mask = _coupled_cell(am1, am2, self.lmax_mask)[0]
+        if fl2 is fl1:
+            self.pcl_mask = self.pcl_mask - fl1.Nw
+        self.wawb = 0.0
+        if self.norm_type:
+            npix = fl1.minfo.npix
+            self.wawb = float(torch.sum(fl1.mask * fl2.mask)) * (4 * np.pi / npix) / (4 * np.pi)
+        self.blocks = coupling_blocks(self.pcl_mask, self.lmax)
+        self.layout = _layout(self.ncls, self.spin1, self.spin2)
+        self._postprocess()
+
+    def _postprocess(self):
+        dev = self.blocks["00"].device
+        d = self.bins._device(dev)
+        beam = (self.beam1 * self.beam2).to(dev)
+        tin = (beam[d["ells"].long()] * d["inv_f"]).contiguous()
+        nb, ncls = self.nbands, self.ncls
+        st = _lib.stream_ptr(dev)
+        binned = {}
+        for name in {b for b, _, _ in self.layout}:
+            out = torch.empty((nb, nb), dtype=torch.float64, device=dev)
+            _lib.call("gm_bin_block", _lib.ptr(self.blocks[name]), self.lmax, _lib.ptr(d["ells"]), _lib.ptr(d["start"]),
+                      nb, _lib.ptr(d["w"]), _lib.ptr(tin), _lib.ptr(out), st)
+            binned[name] = out
+        mb = torch.zeros((nb, ncls, nb, ncls), dtype=torch.float64, device=dev)
+        for name, (c1, c2), sign in self.layout:
+            mb[:, c1, :, c2] += sign * binned[name]
+        mb = mb.reshape(nb * ncls, nb * ncls)
+        if self.norm_type:
+            mb = self.wawb * torch.eye(nb * ncls, dtype=torch.float64, device=dev)
+        self.mcm_binned = mb
+        self._bpws = None
+
+    # ------------------------------------------------------------------ products
+    def decouple_cell(self, cl_in, cl_bias=None, cl_noise=None):
+        """``(ncls, >= lmax+1)`` coupled spectra -> ``(ncls, nbands)`` bandpowers."""
+        cl = device_tensor(cl_in, device=self.mcm_binned.device)
+        if cl.ndim != 2 or cl.shape[0] != self.ncls or cl.shape[1] < self.lmax + 1:
+            raise ValueError(f"Input power spectrum has wrong shape. Expected ({self.ncls}, {self.lmax + 1}), "
+                             f"but got {tuple(cl.shape)}")
+        total = cl[:, : self.lmax + 1]
+        for name, bias in (("bias", cl_bias), ("noise", cl_noise)):
+            if bias is not None:
+                b = device_tensor(bias, device=cl.device)
+                if b.ndim != 2 or b.shape[0] != self.ncls or b.shape[1] < self.lmax + 1:
+                    raise ValueError(f"Input {name} power spectrum has wrong shape")
+                total = total - b[:, : self.lmax + 1]
+        binned = self.bins.bin_cell(total.contiguous()).as_subclass(torch.Tensor)
+        rhs = binned.T.reshape(-1)
+        sol = torch.linalg.solve(self.mcm_binned, rhs)
+        return wrap(sol.reshape(self.nbands, self.ncls).T.contiguous())
+
+    def couple_cell(self, cl_in):
+        """Theory ``(ncls, >= lmax+1)`` -> expected coupled spectra ``(ncls, lmax+1)``."""
+        cl = device_tensor(cl_in, device=self.mcm_binned.device)
+        if cl.ndim != 2 or cl.shape[0] != self.ncls or cl.shape[1] < self.lmax + 1:
+            raise ValueError(f"Input power spectrum has wrong shape. Expected ({self.ncls}, {self.lmax + 1}), "
+                             f"but got {tuple(cl.shape)}")
+        th = cl[:, : self.lmax + 1] * (self.beam1 * self.beam2)[None, :]
+        out = torch.zeros_like(th)
+        for name, (c1, c2), sign in self.layout:
+            out[c1] += sign * (self.blocks[name] @ th[c2])
+        return wrap(out)
+
+    def get_coupling_matrix(self):
+        """Dense ``(ncls (lmax+1))^2`` MCM in ``l * ncls + c`` ordering."""
+        L, ncls = self.lmax + 1, self.ncls
+        out = torch.zeros((L, ncls, L, ncls), dtype=torch.float64, device=self.mcm_binned.device)
+        for name, (c1, c2), sign in self.layout:
+            out[:, c1, :, c2] += sign * self.blocks[name]
+        return wrap(out.reshape(L * ncls, L * ncls))
+
+    def get_bandpower_windows(self):
+        """``(ncls, nbands, ncls, lmax+1)`` windows ``solve(mcm_binned, W M diag(beam))``."""
+        if self._bpws is None:
+            dev = self.mcm_binned.device
+            L, nb, ncls = self.lmax + 1, self.nbands, self.ncls
+            wbin = torch.zeros((nb, L), dtype=torch.float64, device=dev)
+            d = self.bins._device(dev)
+            bp = torch.as_tensor(self.bins._bpws_np, dtype=torch.long, device=dev)
+            wbin[bp, d["ells"].long()] = d["w"]
+            beam = self.beam1 * self.beam2
+            one = torch.zeros((nb, ncls, L, ncls), dtype=torch.float64, device=dev)
+            for name, (c1, c2), sign in self.layout:
+                one[:, c1, :, c2] += sign * (wbin @ self.blocks[name]) * beam[None, :]
+            sol = torch.linalg.solve(self.mcm_binned, one.reshape(nb * ncls, L * ncls))
+            self._bpws = sol.reshape(nb, ncls, L, ncls).permute(1, 0, 3, 2).contiguous()
+        return wrap(self._bpws)
+
+    def update_beams(self, beam1, beam2):
+        L = self.lmax + 1
+        b1 = device_tensor(beam1, device=self.mcm_binned.device)
+        b2 = device_tensor(beam2, device=self.mcm_binned.device)
+        if b1.shape != (L,) or b2.shape != (L,):
+            raise ValueError(f"The new beams must go up to ell = {self.lmax}")
+        self.beam1, self.beam2 = b1, b2
+        self._postprocess()
+
+    def update_bins(self, bins):
+        if bins.lmax != self.lmax:
+            raise ValueError("The new binning scheme has a different maximum multipole")
+        self.bins = bins
+        self.nbands = bins.get_n_bands()
+        self._postprocess()

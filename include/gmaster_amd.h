@@ -99,8 +99,9 @@ int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t spec_ma
                   const int32_t* seed_exp, const int32_t* nactive, double* partial,
                   int64_t nchunks, void* stream);
 
-/* Sum chunk partials, scale by P and write alm (mode 0: alm = result; 1: alm -= result). */
-int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nchunks, const int32_t* nactive,
+/* Sum the ring-chunk partials of the analysis (chunking is a function of nside), scale by
+ * P and write alm (mode 0: alm = result; 1: alm -= result, the Jacobi update). */
+int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
                      const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
                      void* stream);
 
@@ -123,7 +124,7 @@ int gm_lat_ana_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, in
 
 /* Finish spin s: sum partials, scale, combine +-m into E/B (healpy packing, 2 x nelem).
  * Replaces gmaster/_sht/healpix.py:117-143 and :812-814. */
-int gm_lat_finish_s(const double* partial, int64_t nchunks, const int32_t* nactive, const double* P,
+int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
                     int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream);
 
 /* Synthesis coefficients for spin s from E/B (gmaster/_sht/healpix.py:599-611). */
@@ -148,11 +149,12 @@ int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t 
 
 /* Mode-coupling blocks 0-0, 0-2, 2-2 even / odd in one pass over the Wigner-3j rows:
  *   M[l1][l2] = (2 l2 + 1) sum_l3 W'_l3 (l1 l2 l3; ...)(l1 l2 l3; ...),  W'_l = (2l+1) W_l / (4 pi)
- * window: lmax+1 doubles (the mask pseudo-C_ell, zero above lmax); g: g(p) for p < lmax+2.
+ * window: lmax_window+1 doubles (the mask pseudo-C_ell; lmax_window <= 2 lmax, terms above it
+ * are zero); g: g(p) = Gamma(p+1/2)/(sqrt(pi) Gamma(p+1)) for p <= lmax.
  * Any of m00, m02, m22p, m22m may be NULL.  Replaces gm_coupling_tt
  * (gmaster/_native/cuda/coupling_tt.cu:18-81) and the quadrature of gmaster/workspaces.py:682-732. */
-int gm_coupling(const double* window, const double* g, int64_t lmax, double* m00, double* m02,
-                double* m22p, double* m22m, void* stream);
+int gm_coupling(const double* window, int64_t lmax_window, const double* g, int64_t lmax, double* m00,
+                double* m02, double* m22p, double* m22m, void* stream);
 
 /* Binned block: out[b1][b2] = sum_{l1 in b1, l2 in b2} wout[l1] M[l1][l2] tin[l2]
  * (CSR bands: ells sorted by band, band_start has nbands+1 entries).

@@ -35,26 +35,33 @@ def _d_rows(L, m, s, x):
 
 
 def coupling_blocks(window, lmax):
-    """``(M00, M02, M22p, M22m)`` each ``(lmax+1, lmax+1)`` from ``window`` (>= lmax+1 entries,
-    only the first lmax+1 are used, matching the zero padding above lmax_mask = lmax)."""
+    """``(M00, M02, M22p, M22m)`` each ``(lmax+1, lmax+1)`` from ``window`` (the mask
+    pseudo-C_ell; entries beyond 2 lmax cannot couple and are ignored)."""
     L = lmax + 1
-    W = np.zeros(L)
-    W[: min(L, len(window))] = np.asarray(window, dtype=np.float64)[:L]
-    n = (3 * lmax) // 2 + 2
+    window = np.asarray(window, dtype=np.float64)[: 2 * lmax + 1]
+    Lw = len(window)
+    n = (2 * lmax + Lw) // 2 + 2
     x, w = roots_legendre(n)
     P = _d_rows(L, 0, 0, x)  # d_00 = P_l
-    xi = ((2 * np.arange(L) + 1) * W) @ P
+    Pw = _d_rows(Lw, 0, 0, x)
     d20 = _d_rows(L, 2, 0, x)  # d_{2,0} = d_{0,2}
     d0m2 = _d_rows(L, 0, 2, x)  # d_{0,-2}
     d2m2 = _d_rows(L, 2, 2, x)  # d_{2,-2}  (= d_{-2,2})
     # d_{2,2}(x) = d_{-2,-2}(x) = (-1)^l d_{2,-2}(-x)
     d22 = _d_rows(L, 2, 2, -x) * ((-1.0) ** np.arange(L))[:, None]
-    wx = w * xi
-    col = (2 * np.arange(L) + 1)[None, :] / (8 * math.pi)
-    M00 = (P * wx) @ P.T * col
-    M02 = (d20 * wx) @ d0m2.T * col
-    Mall = (d22 * wx) @ d22.T * col
-    Msig = (d2m2 * wx) @ d2m2.T * col
+    # the node sums cancel by ~|xi| / |M| (up to ~1e3 for a white window), so they are
+    # accumulated in long double
+    ld = np.longdouble
+    wx = w.astype(ld) * (((2 * np.arange(Lw) + 1) * window).astype(ld) @ Pw.astype(ld))
+    col = ((2 * np.arange(L) + 1)[None, :] / (8 * np.pi)).astype(ld)
+
+    def contract(a, b):
+        return ((a.astype(ld) * wx) @ b.T.astype(ld) * col).astype(np.float64)
+
+    M00 = contract(P, P)
+    M02 = contract(d20, d0m2)
+    Mall = contract(d22, d22)
+    Msig = contract(d2m2, d2m2)
     M22p = 0.5 * (Mall + Msig)
     M22m = 0.5 * (Mall - Msig)
     return M00, M02, M22p, M22m
@@ -65,13 +72,12 @@ def coupling_blocks_exact(window, lmax):
     from .wigner3j import threej_exact
 
     L = lmax + 1
-    W = np.zeros(L)
-    W[: min(L, len(window))] = np.asarray(window, dtype=np.float64)[:L]
+    W = np.asarray(window, dtype=np.float64)[: 2 * lmax + 1]
     out = np.zeros((4, L, L))
     for l1 in range(L):
         for l2 in range(L):
             acc = np.zeros(4)
-            for l3 in range(abs(l1 - l2), min(l1 + l2, lmax) + 1):
+            for l3 in range(abs(l1 - l2), min(l1 + l2, len(W) - 1) + 1):
                 w0 = threej_exact(l1, l2, l3, 0, 0, 0)
                 w2 = threej_exact(l1, l2, l3, 2, -2, 0)
                 f = (2 * l3 + 1) * W[l3]

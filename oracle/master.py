@@ -79,3 +79,29 @@ def workspace(pcl_mask, lmax, spin1, spin2, nlb):
     ells, bpws, w = linear_bins(lmax, nlb)
     out_op, th_op = binning_operators(lmax, ells, bpws, w)
     return mcm, binned_mcm(mcm, ncls, out_op, th_op), ncls, (ells, bpws, w)
+
+
+def pipeline_tp(mask, tqu, nside, lmax, nlb, n_iter):
+    """Full T+P MASTER on the oracle: fields (n_iter Jacobi steps), mask pseudo-C_ell,
+    00 / 02 / 22 workspaces, coupled spectra and decoupled bandpowers
+    (``gmaster/field.py:78-315``, ``gmaster/workspaces.py:1668-2147``)."""
+    from .sht import map2alm
+
+    mask = np.asarray(mask, dtype=np.float64)
+    tqu = np.asarray(tqu, dtype=np.float64)
+    aT = map2alm(mask * tqu[0], 0, nside, lmax, n_iter)
+    aM = map2alm(mask, 0, nside, lmax, n_iter)
+    aP = map2alm(mask[None] * tqu[1:3], 2, nside, lmax, n_iter)
+    pcl = coupled_cell(aM, aM, lmax)[0]
+    blocks = coupling_blocks(pcl, lmax)
+    ells, bpws, w = linear_bins(lmax, nlb)
+    out_op, th_op = binning_operators(lmax, ells, bpws, w)
+    res = {"alm_T": aT[0], "alm_mask": aM[0], "alm_E": aP[0], "alm_B": aP[1], "pcl_mask": pcl}
+    for name, (s1, s2, a, b) in {"00": (0, 0, aT, aT), "02": (0, 2, aT, aP), "22": (2, 2, aP, aP)}.items():
+        mcm, ncls = assemble_mcm(blocks, s1, s2)
+        mb = binned_mcm(mcm, ncls, out_op, th_op)
+        cl = coupled_cell(a, b, lmax)
+        res["cl_" + name] = cl
+        res["mcm_binned_" + name] = mb
+        res["bp_" + name] = decouple(mb, cl, lmax, ells, bpws, w, ncls)
+    return res

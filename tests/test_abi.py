@@ -56,7 +56,8 @@ def test_nchunks():
         ("gm_lat_ana_s0", (1, 1, 11, 0, 8, 10, 1, 1, 1, 1, 1, 7, None), 2),  # wrong chunk count
         ("gm_lat_ana_s", (1, 21, 8, 10, 0, 1, 1, 1, 1, 1, 1, 1, None), 1),   # spin 0 on the spin-s entry
         ("gm_lat_tables", (10, 11, 1, 1, 1, None), 1),                      # spin > lmax
-        ("gm_coupling", (None, 1, 10, 1, None, None, None, None), 1),       # null window
+        ("gm_coupling", (None, 10, 1, 10, 1, None, None, None, None), 1),   # null window
+        ("gm_coupling", (1, 21, 1, 10, 1, None, None, None, None), 1),      # window beyond 2 lmax
         ("gm_coupled_cell", (1, 3, 1, 1, 66, 10, 1, 1, None), 1),           # 3 components
     ],
 )

@@ -1,0 +1,60 @@
+"""Host-side logic of gmaster_amd that runs without a GPU."""
+
+import math
+
+import numpy as np
+import pytest
+
+from gmaster_amd import bins, sht
+from oracle import legendre, master
+
+
+@pytest.mark.parametrize("s", [0, 1, 2, 3])
+def test_seed_constants_match_oracle(s):
+    mant, exp = sht._seed_constants(2000, s)
+    for m in list(range(0, 12)) + [100, 999, 1999]:
+        sign, c = legendre.seed_log2(m, s)
+        assert np.sign(mant[m]) == sign
+        assert abs(math.log2(abs(mant[m])) + exp[m] - c) < 1e-13 * max(1.0, abs(c))
+
+
+@pytest.mark.parametrize("nside,spin", [(64, 0), (64, 2), (1024, 0), (1024, 2)])
+def test_polar_skip_prefix_is_conservative(nside, spin):
+    lmax = 3 * nside - 1
+    na = sht._nactive(nside, lmax, spin, True)
+    z, sin = sht._ring_slots(nside, spin)
+    ofs = max(100.0, 0.01 * lmax)
+    b = lmax * sin + ofs
+    mlim = spin * np.abs(z) + np.sqrt(np.maximum(b * b - spin * spin * sin * sin, 0.0))
+    for m in range(0, lmax + 1, max(1, lmax // 50)):
+        needed = np.flatnonzero(mlim >= m)
+        assert na[m] >= (needed.max() + 1 if len(needed) else 0)
+    assert np.all(np.diff(na) <= 0)
+    assert np.all(sht._nactive(nside, lmax, spin, False) == len(z))
+
+
+def test_ring_slot_order_visits_every_ring_once():
+    for spin, n in ((0, 16), (2, 16)):
+        z, _ = sht._ring_slots(n, spin)
+        assert np.all(np.diff(np.abs(z)) >= -1e-15)
+
+
+def test_nmtbin_matches_oracle_linear_bands():
+    b = bins.NmtBin.from_nside_linear(64, 16)
+    ells, bpws, w = master.linear_bins(191, 16)
+    assert b.get_n_bands() == bpws.max() + 1
+    for k in range(b.get_n_bands()):
+        assert np.array_equal(b.get_ell_list(k), ells[bpws == k])
+        assert np.allclose(b.get_weight_list(k), w[bpws == k])
+    assert np.allclose(b.get_effective_ells(), np.bincount(bpws, weights=ells * w))
+
+
+def test_nmtbin_errors_and_dell():
+    with pytest.raises(ValueError):
+        bins.NmtBin(bpws=[], ells=[])
+    b = bins.NmtBin.from_lmax_linear(40, 4, is_Dell=True)
+    f = b.get_fell_list(0)
+    ell = b.get_ell_list(0)
+    assert np.allclose(f, ell * (ell + 1) / (2 * np.pi))
+    e = bins.NmtBin.from_edges([2, 10, 20], [10, 20, 41])
+    assert e.get_n_bands() == 3 and e.lmax == 40
