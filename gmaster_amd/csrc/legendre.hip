@@ -49,7 +49,8 @@ __device__ inline void seed_lane(double smant, int sexp, int a, int b, double sh
 // ------------------------------------------------------------------ block reduction
 // acc[NV] per thread -> out[0..NV) = sum over all NW*64 threads; one barrier per call
 // (double-buffered LDS: `buf` alternates between calls).  Threads o*TPO .. (o+1)*TPO-1
-// sum output o, TPO = 64*NW / NV, each over NV consecutive thread slots.
+// sum output o, TPO = 64*NW / NV, each over NV thread slots strided by TPO (bank-conflict
+// free), then combine with TPO-lane shuffles.
 template <int NV, int NW>
 __device__ inline void block_reduce(const double (&acc)[NV], double* lds, int buf, double* out) {
     constexpr int T = 64 * NW;
@@ -60,10 +61,11 @@ __device__ inline void block_reduce(const double (&acc)[NV], double* lds, int bu
     for (int v = 0; v < NV; ++v) base[v * T + tid] = acc[v];
     __syncthreads();
     const int o = tid / TPO, sl = tid % TPO;
-    const double* src = base + o * T + sl * NV;
+    // thread slot sl sums lanes sl, sl + TPO, ... : consecutive threads read consecutive words
+    const double* src = base + o * T + sl;
     double s = 0.0;
 #pragma unroll
-    for (int u = 0; u < NV; ++u) s += src[u];
+    for (int u = 0; u < NV; ++u) s += src[u * TPO];
 #pragma unroll
     for (int msk = 1; msk < TPO; msk <<= 1) s += __shfl_xor(s, msk);
     if (sl == 0) out[o] = s;
@@ -144,8 +146,19 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
 
     const double* al = alpha + table_base(m, L) - m;
     double* part = partial + ((int64_t)c * tsize + table_base(m, L) - m) * (2 * K);
+    // mu0 holds degree l (even step), mu1 degree l - 1; each step overwrites the older one,
+    // so a block of 4 degrees returns the roles to where they started (no register moves).
+    double an[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) an[i] = al[m + i + 1];
     int buf = 0;
     for (int l0 = m; l0 < L; l0 += 4, buf ^= 1) {
+        double a[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a[i] = an[i];
+            an[i] = al[l0 + 4 + i + 1];  // prefetch the next block's coefficients
+        }
         double acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
@@ -154,19 +167,18 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
             // l0 + m is even, so degree l0 + i pairs with the even (E) combination for even i
 #pragma unroll
             for (int t = 0; t < R; ++t) {
+                const double cur = (i & 1) ? mu1[t] : mu0[t];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const double2 gv = (i & 1) ? gO[t][k] : gE[t][k];
-                    acc[i * 2 * K + 2 * k] = fma(mu0[t], gv.x, acc[i * 2 * K + 2 * k]);
-                    acc[i * 2 * K + 2 * k + 1] = fma(mu0[t], gv.y, acc[i * 2 * K + 2 * k + 1]);
+                    acc[i * 2 * K + 2 * k] = fma(cur, gv.x, acc[i * 2 * K + 2 * k]);
+                    acc[i * 2 * K + 2 * k + 1] = fma(cur, gv.y, acc[i * 2 * K + 2 * k + 1]);
                 }
             }
-            const double a = al[l0 + i + 1];
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                const double nu = fma(x[t] * a, mu0[t], -mu1[t]);
-                mu1[t] = mu0[t];
-                mu0[t] = nu;
+                if (i & 1) mu0[t] = fma(x[t] * a[i], mu1[t], -mu0[t]);
+                else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
             }
         }
         rescale<R>(mu0, mu1, kx, anyinact);
@@ -217,7 +229,16 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
         anyinact = __any(anyinact);
         const double* al = alpha + table_base(m, L) - m;
         const double2* cf = coef + (table_base(m, L) - m) * K;
+        double an[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) an[i] = al[m + i + 1];
         for (int l0 = m; l0 < L; l0 += 4) {
+            double a[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = an[i];
+                an[i] = al[l0 + 4 + i + 1];
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 double2 cv[K];
@@ -225,19 +246,18 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
                 for (int k = 0; k < K; ++k) cv[k] = cf[(int64_t)(l0 + i) * K + k];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
+                    const double cur = (i & 1) ? mu1[t] : mu0[t];
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
                         double2& acc = (i & 1) ? aO[t][k] : aE[t][k];
-                        acc.x = fma(mu0[t], cv[k].x, acc.x);
-                        acc.y = fma(mu0[t], cv[k].y, acc.y);
+                        acc.x = fma(cur, cv[k].x, acc.x);
+                        acc.y = fma(cur, cv[k].y, acc.y);
                     }
                 }
-                const double a = al[l0 + i + 1];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    const double nu = fma(x[t] * a, mu0[t], -mu1[t]);
-                    mu1[t] = mu0[t];
-                    mu0[t] = nu;
+                    if (i & 1) mu0[t] = fma(x[t] * a[i], mu1[t], -mu0[t]);
+                    else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
                 }
             }
             rescale<R>(mu0, mu1, kx, anyinact);
@@ -317,8 +337,22 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
     const double* al = alpha + table_base(m, L) - m;
     const double* ga = gamma + table_base(m, L) - m;
     double* part = partial + ((int64_t)c * tsize + table_base(m, L) - m) * 4;
+    double an[4], gn[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        an[i] = al[M + i + 1];
+        gn[i] = ga[M + i + 1];
+    }
     int buf = 0;
     for (int l0 = M; l0 < L; l0 += 4, buf ^= 1) {
+        double a[4], gg[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a[i] = an[i];
+            gg[i] = gn[i];
+            an[i] = al[l0 + 4 + i + 1];
+            gn[i] = ga[l0 + 4 + i + 1];
+        }
         double acc[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) acc[v] = 0.0;
@@ -326,17 +360,16 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                acc[4 * i + 0] = fma(mu0[t], gD[t].x, acc[4 * i + 0]);
-                acc[4 * i + 1] = fma(mu0[t], gD[t].y, acc[4 * i + 1]);
-                acc[4 * i + 2] = fma(mu0[t], gM[t].x, acc[4 * i + 2]);
-                acc[4 * i + 3] = fma(mu0[t], gM[t].y, acc[4 * i + 3]);
+                const double cur = (i & 1) ? mu1[t] : mu0[t];
+                acc[4 * i + 0] = fma(cur, gD[t].x, acc[4 * i + 0]);
+                acc[4 * i + 1] = fma(cur, gD[t].y, acc[4 * i + 1]);
+                acc[4 * i + 2] = fma(cur, gM[t].x, acc[4 * i + 2]);
+                acc[4 * i + 3] = fma(cur, gM[t].y, acc[4 * i + 3]);
             }
-            const double a = al[l0 + i + 1], gm_ = ga[l0 + i + 1];
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                const double nu = fma(fma(x[t], a, gm_), mu0[t], -mu1[t]);
-                mu1[t] = mu0[t];
-                mu0[t] = nu;
+                if (i & 1) mu0[t] = fma(fma(x[t], a[i], gg[i]), mu1[t], -mu0[t]);
+                else mu1[t] = fma(fma(x[t], a[i], gg[i]), mu0[t], -mu1[t]);
             }
         }
         rescale<R>(mu0, mu1, kx, anyinact);
@@ -389,23 +422,36 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
         const double* al = alpha + table_base(m, L) - m;
         const double* ga = gamma + table_base(m, L) - m;
         const double2* cf = coef + (table_base(m, L) - m) * 2;
+        double an[4], gn[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            an[i] = al[M + i + 1];
+            gn[i] = ga[M + i + 1];
+        }
         for (int l0 = M; l0 < L; l0 += 4) {
+            double a[4], gg[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = an[i];
+                gg[i] = gn[i];
+                an[i] = al[l0 + 4 + i + 1];
+                gn[i] = ga[l0 + 4 + i + 1];
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const double2 cd = cf[(int64_t)(l0 + i) * 2], cm = cf[(int64_t)(l0 + i) * 2 + 1];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    aD[t].x = fma(mu0[t], cd.x, aD[t].x);
-                    aD[t].y = fma(mu0[t], cd.y, aD[t].y);
-                    aM[t].x = fma(mu0[t], cm.x, aM[t].x);
-                    aM[t].y = fma(mu0[t], cm.y, aM[t].y);
+                    const double cur = (i & 1) ? mu1[t] : mu0[t];
+                    aD[t].x = fma(cur, cd.x, aD[t].x);
+                    aD[t].y = fma(cur, cd.y, aD[t].y);
+                    aM[t].x = fma(cur, cm.x, aM[t].x);
+                    aM[t].y = fma(cur, cm.y, aM[t].y);
                 }
-                const double a = al[l0 + i + 1], gm_ = ga[l0 + i + 1];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    const double nu = fma(fma(x[t], a, gm_), mu0[t], -mu1[t]);
-                    mu1[t] = mu0[t];
-                    mu0[t] = nu;
+                    if (i & 1) mu0[t] = fma(fma(x[t], a[i], gg[i]), mu1[t], -mu0[t]);
+                    else mu1[t] = fma(fma(x[t], a[i], gg[i]), mu0[t], -mu1[t]);
                 }
             }
             rescale<R>(mu0, mu1, kx, anyinact);
@@ -565,26 +611,21 @@ __global__ void k_prep_s(const double2* __restrict__ ae, const double2* __restri
 }
 
 // ------------------------------------------------------------------ configs
+// Large geometries use 4-wave workgroups with R rings per lane: R = 6 for spin 0 (two
+// maps' E/O coefficients per ring keep a wave under 256 VGPRs, i.e. two waves per SIMD),
+// R = 8 for spin s.
 LatConfig lat_config(int64_t nside, int64_t spin) {
     const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
     LatConfig cfg;
     if (nrings <= 128) cfg = {2, 1, 128, 0};
     else if (nrings <= 512) cfg = {4, 2, 512, 0};
+    else if (spin == 0) cfg = {6, 4, 1536, 0};
     else cfg = {8, 4, 2048, 0};
     cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
     return cfg;
 }
 
-// synthesis: 4-wave workgroups
-LatConfig syn_config(int64_t nside, int64_t spin) {
-    const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
-    LatConfig cfg;
-    if (nrings <= 128) cfg = {2, 1, 128, 0};
-    else if (nrings <= 512) cfg = {4, 2, 512, 0};
-    else cfg = {8, 4, 2048, 0};
-    cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
-    return cfg;
-}
+LatConfig syn_config(int64_t nside, int64_t spin) { return lat_config(nside, spin); }
 
 int check_lat(int64_t nside, int64_t lmax, int64_t spin) {
     if (nside < 1 || (nside & (nside - 1))) return fail(GM_E_ARG, "nside must be a power of 2");
@@ -612,12 +653,6 @@ extern "C" int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* 
     return check_launch("gm_lat_tables");
 }
 
-#define GM_LAUNCH_CFG(KERNEL_T, cfg, ...)                                                        \
-    do {                                                                                         \
-        if ((cfg).R == 2) hipLaunchKernelGGL((KERNEL_T<2, 1>), __VA_ARGS__);                     \
-        else if ((cfg).R == 4) hipLaunchKernelGGL((KERNEL_T<4, 2>), __VA_ARGS__);                \
-        else hipLaunchKernelGGL((KERNEL_T<8, 4>), __VA_ARGS__);                                  \
-    } while (0)
 
 extern "C" int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
                              int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
@@ -639,11 +674,11 @@ extern "C" int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int6
     if (nmaps == 1) {
         if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s0<1, 2, 1>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
         else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s0<1, 4, 2>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
-        else hipLaunchKernelGGL((k_ana_s0<1, 8, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        else hipLaunchKernelGGL((k_ana_s0<1, 6, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
     } else {
         if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s0<2, 2, 1>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
         else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s0<2, 4, 2>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
-        else hipLaunchKernelGGL((k_ana_s0<2, 8, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        else hipLaunchKernelGGL((k_ana_s0<2, 6, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
     }
     return check_launch("gm_lat_ana_s0");
 }
@@ -692,11 +727,11 @@ extern "C" int gm_lat_syn_s0(const double* coef, int64_t nmaps, int64_t nside, i
     if (nmaps == 1) {
         if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s0<1, 2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
         else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s0<1, 4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-        else hipLaunchKernelGGL((k_syn_s0<1, 8, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+        else hipLaunchKernelGGL((k_syn_s0<1, 6, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
     } else {
         if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s0<2, 2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
         else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s0<2, 4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-        else hipLaunchKernelGGL((k_syn_s0<2, 8, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
+        else hipLaunchKernelGGL((k_syn_s0<2, 6, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
     }
     return check_launch("gm_lat_syn_s0");
 }

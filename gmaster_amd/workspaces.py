@@ -51,7 +51,8 @@ def coupling_blocks(window, lmax):
     dev = w.device
     blocks = {k: torch.empty((L, L), dtype=torch.float64, device=dev) for k in ("00", "02", "22p", "22m")}
     g = _g_table(L + 1, dev)
-    _lib.call("gm_coupling", _lib.ptr(w), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(blocks["00"]),
+    wp = (w * (2 * torch.arange(len(w), dtype=torch.float64, device=dev) + 1) / (4 * np.pi)).contiguous()
+    _lib.call("gm_coupling", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(blocks["00"]),
               _lib.ptr(blocks["02"]), _lib.ptr(blocks["22p"]), _lib.ptr(blocks["22m"]), _lib.stream_ptr(dev))
     if len(_COUPLING_CACHE) >= 2:
         _COUPLING_CACHE.clear()

@@ -148,10 +148,10 @@ int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t 
                     int64_t lmax, double* cl, double* work, void* stream);
 
 /* Mode-coupling blocks 0-0, 0-2, 2-2 even / odd in one pass over the Wigner-3j rows:
- *   M[l1][l2] = (2 l2 + 1) sum_l3 W'_l3 (l1 l2 l3; ...)(l1 l2 l3; ...),  W'_l = (2l+1) W_l / (4 pi)
- * window: lmax_window+1 doubles (the mask pseudo-C_ell; lmax_window <= 2 lmax, terms above it
- * are zero); g: g(p) = Gamma(p+1/2)/(sqrt(pi) Gamma(p+1)) for p <= lmax.
- * Any of m00, m02, m22p, m22m may be NULL.  Replaces gm_coupling_tt
+ *   M[l1][l2] = (2 l2 + 1) sum_l3 W'_l3 (l1 l2 l3; ...)(l1 l2 l3; ...)
+ * window: the weighted mask pseudo-C_ell W'_l = (2l+1) W_l / (4 pi), lmax_window+1 doubles
+ * (lmax_window <= 2 lmax; terms above it are zero); g: g(p) = Gamma(p+1/2)/(sqrt(pi) Gamma(p+1))
+ * for p <= lmax.  Any of m00, m02, m22p, m22m may be NULL.  Replaces gm_coupling_tt
  * (gmaster/_native/cuda/coupling_tt.cu:18-81) and the quadrature of gmaster/workspaces.py:682-732. */
 int gm_coupling(const double* window, int64_t lmax_window, const double* g, int64_t lmax, double* m00,
                 double* m02, double* m22p, double* m22m, void* stream);

@@ -41,7 +41,7 @@ def test_table_size_matches_layout():
 
 def test_nchunks():
     assert _lib.query("gm_lat_nchunks", 64, 0) == 1
-    assert _lib.query("gm_lat_nchunks", 4096, 0) == 4
+    assert _lib.query("gm_lat_nchunks", 4096, 0) == 6
     assert _lib.query("gm_lat_nchunks", 4096, 2) == 8
 
 
