@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--nlb", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU sample time")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (scales PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -95,6 +97,14 @@ def main():
     tqu = workload.tqu_maps(nside, lmax, seed=4321, device=dev)
     torch.cuda.synchronize()
     log(f"inputs ready in {time.time() - t0:.1f}s (fsky={float(mask.mean()):.3f})")
+
+    if args.calibrate_copy:
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 1 GiB read + 1 GiB written")
 
     for i in range(args.warmup):
         t = time.time()

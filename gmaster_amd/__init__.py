@@ -21,7 +21,7 @@ from .utils import (
     mask_apodization,
     synfast_spherical,
 )
-from .workspaces import NmtWorkspace, compute_coupled_cell
+from .workspaces import NmtWorkspace, compute_coupled_cell, get_master_coefficients
 
 __all__ = [
     "GmArray",
@@ -33,6 +33,7 @@ __all__ = [
     "alm2map",
     "compute_coupled_cell",
     "get_default_params",
+    "get_master_coefficients",
     "map2alm",
     "map2alm_pair",
     "mask_apodization",

@@ -35,6 +35,8 @@ SIGNATURES = {
     "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_coupled_cell": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p],
     "gm_coupling": [_p, _i64, _p, _i64, _p, _p, _p, _p, _p],
+    "gm_coupling_pairs": [_p, _i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _p, _p],
+    "gm_toeplitz": [_p, _i64, _i64, _i64, _i64, _p, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
     "gm_table_size": [_i64],

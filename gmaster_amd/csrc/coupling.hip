@@ -24,12 +24,9 @@
 namespace gm {
 namespace {
 
-__global__ void k_coupling(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g, int L,
-                           double* __restrict__ m00, double* __restrict__ m02, double* __restrict__ m22p,
-                           double* __restrict__ m22m) {
-    const int l1 = blockIdx.y;
-    const int l2 = l1 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (l2 >= L) return;
+// Row sums for one pair l1 <= l2 (see above); results in s[0..3] = (a00, a02, aE, aO).
+__device__ __forceinline__ void couple_pair(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g,
+                                            int l1, int l2, double s[4]) {
     const int d = l2 - l1;
     const double S2 = (double)(l1 + l2 + 1) * (double)(l1 + l2 + 1);
     const double d2 = (double)d * (double)d;
@@ -110,18 +107,107 @@ __global__ void k_coupling(const double* __restrict__ Wp, int lmax_w, const doub
         }
     }
 done:
+    s[0] = a00;
+    s[1] = a02;
+    s[2] = aE;
+    s[3] = aO;
+}
+
+__device__ __forceinline__ void store_pair(int l1, int l2, int L, const double s[4], double* __restrict__ m00,
+                                           double* __restrict__ m02, double* __restrict__ m22p,
+                                           double* __restrict__ m22m) {
     const double c2 = 2.0 * l2 + 1.0, c1 = 2.0 * l1 + 1.0;
     const int64_t i12 = (int64_t)l1 * L + l2, i21 = (int64_t)l2 * L + l1;
-    if (m00) { m00[i12] = c2 * a00; m00[i21] = c1 * a00; }
-    if (m02) { m02[i12] = c2 * a02; m02[i21] = c1 * a02; }
-    if (m22p) { m22p[i12] = c2 * aE; m22p[i21] = c1 * aE; }
-    if (m22m) { m22m[i12] = c2 * aO; m22m[i21] = c1 * aO; }
+    if (m00) { m00[i12] = c2 * s[0]; m00[i21] = c1 * s[0]; }
+    if (m02) { m02[i12] = c2 * s[1]; m02[i21] = c1 * s[1]; }
+    if (m22p) { m22p[i12] = c2 * s[2]; m22p[i21] = c1 * s[2]; }
+    if (m22m) { m22m[i12] = c2 * s[3]; m22m[i21] = c1 * s[3]; }
+}
+
+__global__ void k_coupling(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g, int L,
+                           double* __restrict__ m00, double* __restrict__ m02, double* __restrict__ m22p,
+                           double* __restrict__ m22m) {
+    const int l1 = blockIdx.y;
+    const int l2 = l1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (l2 >= L) return;
+    double s[4];
+    couple_pair(Wp, lmax_w, g, l1, l2, s);
+    store_pair(l1, l2, L, s, m00, m02, m22p, m22m);
+}
+
+// Only the listed pairs (lo[k] <= hi[k]); every other element of the outputs is left as it was.
+__global__ void k_coupling_pairs(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g, int L,
+                                 const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, int64_t npairs,
+                                 double* __restrict__ m00, double* __restrict__ m02, double* __restrict__ m22p,
+                                 double* __restrict__ m22m) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs) return;
+    const int l1 = lo[k], l2 = hi[k];
+    double s[4];
+    couple_pair(Wp, lmax_w, g, l1, l2, s);
+    store_pair(l1, l2, L, s, m00, m02, m22p, m22m);
+}
+
+// Toeplitz approximation of one coupling block (Louis et al. 2020, NaMaster's fill order as
+// in gmaster/workspaces.py:1141-1194), on the master coefficients S = M / (2 l' + 1).  The
+// reference applies a sequence of overlapping fills to the lower triangle and then mirrors
+// it; here each output element evaluates the last fill that covers it:
+//   pure(i, j) = c_t[(i - j + lt) mod n] sqrt(D_i D_j),  c_t[k] = S[k][lt] / sqrt(D_k D_lt)
+//   j <= le or i - j <= dl -> exact;  i, j >= lt -> pure;  i == j -> exact;
+//   1 <= i - j <= dl with j <= lt -> exact;
+//   le <= j <= lt, i >= n - 1 - lt + j -> c_e[le + i - j] sqrt(D_i D_j),  c_e[k] = S[k][le] / sqrt(D_k D_le)
+// for i = max(row, col), j = min(row, col); band and diagonal read S from the upper
+// triangle exactly as the reference's diag(S, offset) does.
+__global__ void k_toeplitz(const double* __restrict__ M, int n, int lt, int le, int dl, double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b >= n) return;
+    const int i = a > b ? a : b, j = a > b ? b : a;
+    auto S = [&](int r, int c) { return M[(int64_t)r * n + c] / (2.0 * c + 1.0); };
+    auto D = [&](int k) { return fabs(S(k, k)); };
+    const double Di = D(i), Dj = D(j);
+    double v;
+    {
+        const int k = (i - j + lt) % n;
+        const double den = sqrt(D(k) * D(lt));
+        v = (den > 0.0 ? S(k, lt) / den : 0.0) * sqrt(Di * Dj);
+    }
+    if (j <= le) v = S(i, j);
+    if (i - j <= dl) v = (i == j) ? S(i, i) : S(j, i);
+    if (i >= lt && j >= lt) {
+        const int k = (i - j + lt) % n;
+        const double den = sqrt(D(k) * D(lt));
+        v = (den > 0.0 ? S(k, lt) / den : 0.0) * sqrt(Di * Dj);
+    }
+    if (i == j) v = S(i, i);
+    if (i - j >= 1 && i - j <= dl && j <= lt) v = S(j, i);
+    if (j >= le && j <= lt && i >= n - 1 - lt + j) {
+        const int k = le + i - j;
+        const double Dk = D(k);
+        const double den = sqrt(Dk * D(le));
+        const double ce = Dk > 0.0 ? S(k, le) / den : 0.0;  // guarded by D_k only, as the reference
+        v = ce * sqrt(Dj * Di);
+    }
+    out[(int64_t)a * n + b] = v * (2.0 * b + 1.0);
 }
 
 }  // namespace
 }  // namespace gm
 
 using namespace gm;
+
+extern "C" int gm_toeplitz(const double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exact, int64_t dl_band,
+                           double* out, void* stream) {
+    if (!M || !out) return fail(GM_E_ARG, "gm_toeplitz: null pointer");
+    if (M == out) return fail(GM_E_ARG, "gm_toeplitz: out must not alias M");
+    if (lmax < 1 || l_toeplitz <= 0 || l_exact <= 0 || dl_band < 0 || l_exact > l_toeplitz ||
+        l_toeplitz >= lmax || dl_band >= lmax)
+        return fail(GM_E_ARG, "gm_toeplitz: need 0 < l_exact <= l_toeplitz < lmax and 0 <= dl_band < lmax");
+    const int n = (int)lmax + 1;
+    hipLaunchKernelGGL(k_toeplitz, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, (hipStream_t)stream,
+                       M, n, (int)l_toeplitz, (int)l_exact, (int)dl_band, out);
+    return check_launch("gm_toeplitz");
+}
 
 extern "C" int gm_coupling(const double* window, int64_t lmax_window, const double* g, int64_t lmax, double* m00,
                            double* m02, double* m22p, double* m22m, void* stream) {
@@ -134,4 +220,21 @@ extern "C" int gm_coupling(const double* window, int64_t lmax_window, const doub
     hipLaunchKernelGGL(k_coupling, dim3((unsigned)((L + bx - 1) / bx), (unsigned)L), dim3(bx), 0,
                        (hipStream_t)stream, window, (int)lmax_window, g, L, m00, m02, m22p, m22m);
     return check_launch("gm_coupling");
+}
+
+extern "C" int gm_coupling_pairs(const double* window, int64_t lmax_window, const double* g, int64_t lmax,
+                                 const int32_t* lo, const int32_t* hi, int64_t npairs, double* m00, double* m02,
+                                 double* m22p, double* m22m, void* stream) {
+    if (!window || !g || (npairs > 0 && (!lo || !hi))) return fail(GM_E_ARG, "gm_coupling_pairs: null pointer");
+    if (lmax < 0 || lmax > 65535) return fail(GM_E_ARG, "gm_coupling_pairs: bad lmax %lld", (long long)lmax);
+    if (!m00 && !m02 && !m22p && !m22m) return fail(GM_E_ARG, "gm_coupling_pairs: no output requested");
+    if (lmax_window < 0 || lmax_window > 2 * lmax)
+        return fail(GM_E_ARG, "gm_coupling_pairs: need 0 <= lmax_window <= 2 lmax");
+    if (npairs < 0) return fail(GM_E_ARG, "gm_coupling_pairs: negative pair count");
+    if (npairs == 0) return GM_OK;
+    const int L = (int)lmax + 1;
+    const int bx = 128;
+    hipLaunchKernelGGL(k_coupling_pairs, dim3((unsigned)((npairs + bx - 1) / bx)), dim3(bx), 0, (hipStream_t)stream,
+                       window, (int)lmax_window, g, L, lo, hi, npairs, m00, m02, m22p, m22m);
+    return check_launch("gm_coupling_pairs");
 }

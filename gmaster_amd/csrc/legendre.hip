@@ -232,18 +232,29 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
         double an[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) an[i] = al[m + i + 1];
+        double2 cn[4][K];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < K; ++k) cn[i][k] = cf[(int64_t)(m + i) * K + k];
         for (int l0 = m; l0 < L; l0 += 4) {
             double a[4];
+            double2 cb[4][K];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 a[i] = an[i];
                 an[i] = al[l0 + 4 + i + 1];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    cb[i][k] = cn[i][k];
+                    cn[i][k] = cf[(int64_t)(l0 + 4 + i) * K + k];  // next block (zero padded)
+                }
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 double2 cv[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) cv[k] = cf[(int64_t)(l0 + i) * K + k];
+                for (int k = 0; k < K; ++k) cv[k] = cb[i][k];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
                     const double cur = (i & 1) ? mu1[t] : mu0[t];
@@ -428,18 +439,29 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
             an[i] = al[M + i + 1];
             gn[i] = ga[M + i + 1];
         }
+        double2 cdn[4], cmn[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            cdn[i] = cf[(int64_t)(M + i) * 2];
+            cmn[i] = cf[(int64_t)(M + i) * 2 + 1];
+        }
         for (int l0 = M; l0 < L; l0 += 4) {
             double a[4], gg[4];
+            double2 cdb[4], cmb[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 a[i] = an[i];
                 gg[i] = gn[i];
                 an[i] = al[l0 + 4 + i + 1];
                 gn[i] = ga[l0 + 4 + i + 1];
+                cdb[i] = cdn[i];
+                cmb[i] = cmn[i];
+                cdn[i] = cf[(int64_t)(l0 + 4 + i) * 2];  // next block (zero padded)
+                cmn[i] = cf[(int64_t)(l0 + 4 + i) * 2 + 1];
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const double2 cd = cf[(int64_t)(l0 + i) * 2], cm = cf[(int64_t)(l0 + i) * 2 + 1];
+                const double2 cd = cdb[i], cm = cmb[i];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
                     const double cur = (i & 1) ? mu1[t] : mu0[t];

@@ -6,6 +6,8 @@ Restates ``gmaster/workspaces.py``:
 * the mode-coupling blocks 00, 02 (TE), 22+ / 22- (``:352-517, 878-924``) -- one pass of
   the Wigner-3j kernel ``gm_coupling`` produces all four, so workspaces built from the
   same mask window share it (cached on the window's content);
+* the Toeplitz approximation (``:520-590, :1125-1194, :1834-1840``) -- kernels
+  ``gm_coupling_pairs`` (exact entries only) and ``gm_toeplitz`` (the fill);
 * the slot / sign layout of the blocks (``:1851-1903``), binning with weights, f_ell and
   beams (``:1452-1470, :292-310``) -- kernel ``gm_bin_block``;
 * ``decouple_cell`` (``:2106-2147``): binned spectra (``gm_bin_cell``) and an FP64 LU
@@ -36,24 +38,74 @@ def _g_table(n, device):
     return t
 
 
-def coupling_blocks(window, lmax):
+def toeplitz_sanity(l_toeplitz, l_exact, dl_band, lmax, fields=()):
+    """Parameter checks of the Toeplitz approximation (``gmaster/workspaces.py:1125-1138``)."""
+    if l_toeplitz <= 0:
+        return
+    if any(getattr(f, "pure_e", False) or getattr(f, "pure_b", False) for f in fields):
+        raise ValueError("Can't use Toeplitz approximation with purification.")
+    if l_exact <= 0 or dl_band < 0:
+        raise ValueError("`l_exact` and `dl_band` must be positive numbers")
+    if l_exact > l_toeplitz:
+        raise ValueError("`l_exact` must be `<= l_toeplitz")
+    if l_toeplitz >= lmax or l_exact >= lmax or dl_band >= lmax:
+        raise ValueError("`l_toeplitz`, `l_exact` and `dl_band` must be smaller than `l_max`")
+
+
+def _toeplitz_pairs(L, l_toeplitz, l_exact, dl_band, device):
+    """``(lo, hi)`` int32: the upper-triangle pairs whose exact values the Toeplitz fill reads
+    (diagonal, column l_toeplitz, columns <= l_exact, dl_band diagonals;
+    ``gmaster/workspaces.py:520-539``), each once."""
+    l = np.arange(L, dtype=np.int64)
+    parts = [np.stack([l, np.full(L, l_toeplitz)])]
+    for c in range(l_exact + 1):
+        parts.append(np.stack([l, np.full(L, c)]))
+    for off in range(dl_band + 1):
+        parts.append(np.stack([l[: L - off], l[: L - off] + off]))
+    rc = np.concatenate(parts, axis=1)
+    lo, hi = np.minimum(rc[0], rc[1]), np.maximum(rc[0], rc[1])
+    enc = np.unique(lo * L + hi)
+    return (torch.as_tensor((enc // L).astype(np.int32), device=device),
+            torch.as_tensor((enc % L).astype(np.int32), device=device))
+
+
+def coupling_blocks(window, lmax, toeplitz=None):
     """``{"00", "02", "22p", "22m"}`` coupling blocks ``(lmax+1, lmax+1)`` for a mask window.
 
-    ``window`` is the mask pseudo-C_ell (entries above 2 lmax cannot couple).  Results are
-    cached on (lmax, window bytes) so the 00, 02 and 22 workspaces of one mask share a pass.
+    ``window`` is the mask pseudo-C_ell (entries above 2 lmax cannot couple).  With
+    ``toeplitz = (l_toeplitz, l_exact, dl_band)`` only the entries the Toeplitz fill reads
+    are evaluated (``gm_coupling_pairs``) and every block is then filled (``gm_toeplitz``),
+    as ``gmaster/workspaces.py:1834-1840`` does to the TE / EE blocks and ``:542-590`` to the
+    TT block.  Results are cached on (lmax, Toeplitz parameters, window bytes) so the 00, 02
+    and 22 workspaces of one mask share a pass.
     """
     w = device_tensor(window)[: 2 * lmax + 1].contiguous()
-    key = (int(lmax), str(w.device), hashlib.sha1(w.cpu().numpy().tobytes()).hexdigest())
+    tkey = tuple(int(v) for v in toeplitz) if toeplitz is not None and toeplitz[0] > 0 else None
+    key = (int(lmax), tkey, str(w.device), hashlib.sha1(w.cpu().numpy().tobytes()).hexdigest())
     hit = _COUPLING_CACHE.get(key)
     if hit is not None:
         return hit
     L = lmax + 1
     dev = w.device
-    blocks = {k: torch.empty((L, L), dtype=torch.float64, device=dev) for k in ("00", "02", "22p", "22m")}
+    names = ("00", "02", "22p", "22m")
     g = _g_table(L + 1, dev)
     wp = (w * (2 * torch.arange(len(w), dtype=torch.float64, device=dev) + 1) / (4 * np.pi)).contiguous()
-    _lib.call("gm_coupling", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(blocks["00"]),
-              _lib.ptr(blocks["02"]), _lib.ptr(blocks["22p"]), _lib.ptr(blocks["22m"]), _lib.stream_ptr(dev))
+    stream = _lib.stream_ptr(dev)
+    if tkey is None:
+        blocks = {k: torch.empty((L, L), dtype=torch.float64, device=dev) for k in names}
+        _lib.call("gm_coupling", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, *(_lib.ptr(blocks[k]) for k in names),
+                  stream)
+    else:
+        lt, le, dl = tkey
+        lo, hi = _toeplitz_pairs(L, lt, le, dl, dev)
+        exact = {k: torch.zeros((L, L), dtype=torch.float64, device=dev) for k in names}
+        _lib.call("gm_coupling_pairs", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(lo), _lib.ptr(hi),
+                  lo.numel(), *(_lib.ptr(exact[k]) for k in names), stream)
+        blocks = {}
+        for k in names:
+            blocks[k] = torch.empty((L, L), dtype=torch.float64, device=dev)
+            _lib.call("gm_toeplitz", _lib.ptr(exact[k]), lmax, lt, le, dl, _lib.ptr(blocks[k]), stream)
+        del exact
     if len(_COUPLING_CACHE) >= 2:
         _COUPLING_CACHE.clear()
     _COUPLING_CACHE[key] = blocks
@@ -127,8 +179,7 @@ class NmtWorkspace:
             raise ValueError(f"Maximum multipoles in bins ({bins.lmax}) and fields ({fl1.ainfo.lmax}) are not the same")
         if is_teb:
             raise NotImplementedError("joint TEB workspaces are not supported by this engine")
-        if l_toeplitz > 0:
-            raise NotImplementedError("the Toeplitz approximation is not supported by this engine")
+        toeplitz_sanity(l_toeplitz, l_exact, dl_band, fl1.ainfo.lmax, (fl1, fl2))
         if normalization not in ("MASTER", "FKP"):
             raise ValueError(f"Unknown normalization type {normalization}. Allowed options are 'MASTER' and 'FKP'")
         for f in (fl1, fl2):
@@ -155,7 +206,8 @@ class NmtWorkspace:
         if self.norm_type:
             npix = fl1.minfo.npix
             self.wawb = float(torch.sum(fl1.mask * fl2.mask)) * (4 * np.pi / npix) / (4 * np.pi)
-        self.blocks = coupling_blocks(self.pcl_mask, self.lmax)
+        self.toeplitz = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
+        self.blocks = coupling_blocks(self.pcl_mask, self.lmax, self.toeplitz)
         self.layout = _layout(self.ncls, self.spin1, self.spin2)
         self._postprocess()
 
@@ -252,3 +304,53 @@ class NmtWorkspace:
         self.bins = bins
         self.nbands = bins.get_n_bands()
         self._postprocess()
+
+
+def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any=False, l_toeplitz=-1, l_exact=-1,
+                            dl_band=-1):
+    """Distinct MASTER coupling blocks of a spin pair, divided by ``2 l' + 1``
+    (``gmaster/workspaces.py:1197-1350``): ``"00"``, ``"0s"``, ``"pp"`` (M+) and ``"mm"`` (M-),
+    None where the spin pair has no such block; ``"0s"``, ``"pp"`` and ``"mm"`` carry a
+    leading axis of length 1 (the purification level), and a mask axis before the last two
+    when ``pcl_mask`` is 2D.  Spins 0 and 2 only."""
+    spin1, spin2, lmax = int(spin1), int(spin2), int(lmax)
+    if is_teb and not (spin1 == 0 and spin2 != 0):
+        raise ValueError("is_teb is only valid for spin1=0 and spin2!=0")
+    if pure_any and spin1 != 2 and spin2 != 2:
+        raise ValueError("pure_any is only valid for spin 2")
+    if pure_any:
+        raise NotImplementedError("purified coupling blocks are not provided by this engine")
+    if spin1 not in (0, 2) or spin2 not in (0, 2):
+        raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
+    toeplitz_sanity(l_toeplitz, l_exact, dl_band, lmax)
+    masks = device_tensor(pcl_mask)
+    one_dimensional = masks.ndim == 1
+    if one_dimensional:
+        masks = masks[None]
+    if masks.ndim != 2:
+        raise ValueError("pcl_mask must be one- or two-dimensional")
+    lmax_mask = masks.shape[1] - 1
+    dev = masks.device
+    cols = (2 * torch.arange(lmax + 1, dtype=torch.float64, device=dev) + 1)[None]
+    has_00 = (spin1 == spin2 == 0) or is_teb
+    has_0s = (spin1 != spin2) or is_teb
+    has_ss = (spin1 != 0 and spin2 != 0) or is_teb
+    toe = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
+    per_mask = []
+    for m in masks:
+        padded = torch.zeros(2 * lmax + 1, dtype=torch.float64, device=dev)
+        n = min(len(m), 2 * lmax + 1)
+        padded[:n] = m[:n]
+        b = coupling_blocks(padded, lmax, toe)
+        per_mask.append({"00": b["00"] / cols, "0s": (b["02"] / cols)[None], "pp": (b["22p"] / cols)[None],
+                         "mm": (b["22m"] / cols)[None]})
+    out = {}
+    for name, present in (("00", has_00), ("0s", has_0s), ("pp", has_ss), ("mm", has_ss)):
+        if not present:
+            out[name] = None
+            continue
+        arrays = [v[name] for v in per_mask]
+        out[name] = wrap(arrays[0] if one_dimensional else torch.stack(arrays, dim=-3))
+    out.update(pure_any=False, toeplitz={"l_toeplitz": l_toeplitz, "l_exact": l_exact, "dl_band": dl_band},
+               spins=(spin1, spin2), lmax=lmax, lmax_mask=lmax_mask)
+    return out

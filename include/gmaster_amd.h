@@ -156,6 +156,21 @@ int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t 
 int gm_coupling(const double* window, int64_t lmax_window, const double* g, int64_t lmax, double* m00,
                 double* m02, double* m22p, double* m22m, void* stream);
 
+/* gm_coupling restricted to the pairs (lo[k], hi[k]), lo[k] <= hi[k] <= lmax (int32); writes
+ * both (lo, hi) and (hi, lo) of each requested block and leaves every other element alone.
+ * Replaces the exact-entry evaluation of gmaster/workspaces.py:542-590
+ * (_coupling_matrix_tt_toeplitz over _toeplitz_exact_pairs, :520-539). */
+int gm_coupling_pairs(const double* window, int64_t lmax_window, const double* g, int64_t lmax,
+                      const int32_t* lo, const int32_t* hi, int64_t npairs, double* m00, double* m02,
+                      double* m22p, double* m22m, void* stream);
+
+/* Toeplitz approximation of one coupling block M ((lmax+1)^2, column factor included)
+ * into out (not aliasing M): NaMaster's fill with l_toeplitz, l_exact, dl_band
+ * (0 < l_exact <= l_toeplitz < lmax).  Replaces gmaster/workspaces.py:1141-1194
+ * (_apply_toeplitz) as used at :1834-1840. */
+int gm_toeplitz(const double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exact, int64_t dl_band,
+                double* out, void* stream);
+
 /* Binned block: out[b1][b2] = sum_{l1 in b1, l2 in b2} wout[l1] M[l1][l2] tin[l2]
  * (CSR bands: ells sorted by band, band_start has nbands+1 entries).
  * Replaces gmaster/workspaces.py:292-310 (_BlockMCM.banded). */

@@ -59,6 +59,11 @@ def test_nchunks():
         ("gm_coupling", (None, 10, 1, 10, 1, None, None, None, None), 1),   # null window
         ("gm_coupling", (1, 21, 1, 10, 1, None, None, None, None), 1),      # window beyond 2 lmax
         ("gm_coupled_cell", (1, 3, 1, 1, 66, 10, 1, 1, None), 1),           # 3 components
+        ("gm_coupling_pairs", (1, 20, 1, 10, None, None, 5, 1, None, None, None, None), 1),  # null pair list
+        ("gm_coupling_pairs", (1, 20, 1, 10, 1, 1, -1, 1, None, None, None, None), 1),      # negative count
+        ("gm_toeplitz", (1, 10, 10, 3, 2, 2, None), 1),                     # l_toeplitz = lmax
+        ("gm_toeplitz", (1, 10, 6, 7, 2, 2, None), 1),                      # l_exact > l_toeplitz
+        ("gm_toeplitz", (1, 10, 6, 3, 2, 1, None), 1),                      # out aliases M
     ],
 )
 def test_argument_validation(name, args, code):

@@ -142,3 +142,100 @@ def test_results_convert_with_numpy_asarray():
     f = nmt.NmtField(m, m[None] * 2.0)
     a = np.asarray(f.get_alms())
     assert a.shape == (1, (3 * nside) * (3 * nside + 1) // 2) and a.dtype == np.complex128
+
+
+TOEPLITZ = [(6, 3, 2), (16, 4, 3), (60, 60, 0), (90, 1, 5)]
+
+
+def test_coupling_pairs_match_full_kernel():
+    from gmaster_amd import _lib
+    from gmaster_amd.workspaces import _g_table, _toeplitz_pairs, clear_coupling_cache, coupling_blocks
+
+    clear_coupling_cache()
+    lmax = 95
+    L = lmax + 1
+    window = np.random.default_rng(19).uniform(size=2 * lmax + 1)
+    w = torch.as_tensor(window, device="cuda")
+    full = coupling_blocks(w, lmax)
+    wp = (w * (2 * torch.arange(len(w), dtype=torch.float64, device="cuda") + 1) / (4 * np.pi)).contiguous()
+    lo, hi = _toeplitz_pairs(L, 16, 4, 3, "cuda")
+    out = {k: torch.full((L, L), float("nan"), dtype=torch.float64, device="cuda") for k in ("00", "02", "22p", "22m")}
+    _lib.call("gm_coupling_pairs", _lib.ptr(wp), len(w) - 1, _lib.ptr(_g_table(L + 1, "cuda")), lmax, _lib.ptr(lo),
+              _lib.ptr(hi), lo.numel(), *(_lib.ptr(out[k]) for k in ("00", "02", "22p", "22m")), _lib.stream_ptr("cuda"))
+    torch.cuda.synchronize()
+    sel = torch.zeros((L, L), dtype=torch.bool, device="cuda")
+    sel[lo.long(), hi.long()] = True
+    sel[hi.long(), lo.long()] = True
+    for k in out:
+        assert torch.equal(out[k][sel], full[k][sel]), k
+        assert bool(torch.isnan(out[k][~sel]).all()), k
+
+
+@pytest.mark.parametrize("lt,le,dl", TOEPLITZ)
+def test_toeplitz_kernel_matches_oracle(lt, le, dl):
+    from gmaster_amd import _lib
+    from oracle.toeplitz import apply_to_mcm_block
+
+    lmax = 95
+    window = np.random.default_rng(lt).uniform(size=2 * lmax + 1) / (1.0 + np.arange(2 * lmax + 1))
+    for M in oracle_blocks(window, lmax):
+        Md = torch.as_tensor(M, device="cuda")
+        out = torch.empty_like(Md)
+        _lib.call("gm_toeplitz", _lib.ptr(Md), lmax, lt, le, dl, _lib.ptr(out), _lib.stream_ptr("cuda"))
+        err = _rel(out.cpu(), apply_to_mcm_block(M, lt, le, dl))
+        print(f"toeplitz lt={lt} le={le} dl={dl} rel={err:.2e}", flush=True)
+        assert err < 1e-14
+
+
+@pytest.mark.parametrize("lt,le,dl", TOEPLITZ[:2])
+def test_master_coefficients_with_toeplitz_match_oracle(lt, le, dl):
+    """The reference's own check (tests/test_workspaces.py:89-105) at lmax 12, held against
+    the oracle's exact blocks under the oracle's fill instead of NaMaster."""
+    import gmaster as nmt
+    from gmaster_amd.workspaces import clear_coupling_cache
+    from oracle.toeplitz import apply_toeplitz
+
+    clear_coupling_cache()
+    lmax = 12 if lt < 12 else 31
+    mask_cell = np.random.default_rng(18).uniform(size=lmax + 1)
+    padded = np.zeros(2 * lmax + 1)
+    padded[: lmax + 1] = mask_cell
+    cols = 2 * np.arange(lmax + 1) + 1
+    ref = dict(zip(("00", "0s", "pp", "mm"), (b / cols for b in oracle_blocks(padded, lmax))))
+    for s1, s2, names in ((0, 0, ("00",)), (0, 2, ("0s",)), (2, 2, ("pp", "mm"))):
+        got = nmt.get_master_coefficients(mask_cell, lmax, s1, s2, l_toeplitz=lt, l_exact=le, dl_band=dl)
+        for name in names:
+            g = np.asarray(got[name])
+            g = g if name == "00" else g[0]
+            want = apply_toeplitz(ref[name], lt, le, dl)
+            np.testing.assert_allclose(g, want, rtol=0, atol=2e-13 * np.abs(want).max())
+
+
+def test_workspace_toeplitz_blocks_and_bandpowers():
+    import gmaster as nmt
+    from gmaster_amd.workspaces import clear_coupling_cache
+    from oracle.toeplitz import apply_to_mcm_block
+
+    clear_coupling_cache()
+    fx = np.load(os.path.join(GOLDEN, "pipeline_nside16.npz"))
+    nside = 16
+    mask = fx["mask"]
+    f0 = nmt.NmtField(mask, [fx["tqu"][0]], n_iter=3)
+    f2 = nmt.NmtField(mask, [fx["tqu"][1], fx["tqu"][2]], n_iter=3)
+    b = nmt.NmtBin.from_nside_linear(nside, 4)
+    exact = nmt.NmtWorkspace.from_fields(f0, f2, b)
+    w = nmt.NmtWorkspace.from_fields(f0, f2, b, l_toeplitz=30, l_exact=10, dl_band=8)
+    for name in ("00", "02", "22p", "22m"):
+        ref = apply_to_mcm_block(exact.blocks[name].cpu().numpy(), 30, 10, 8)
+        err = _rel(w.blocks[name].cpu(), ref)
+        print(f"workspace toeplitz block {name} rel={err:.2e}", flush=True)
+        assert err < 1e-13
+    cl = nmt.compute_coupled_cell(f0, f2)
+    bp_t = np.asarray(w.decouple_cell(cl))
+    bp_e = np.asarray(exact.decouple_cell(cl))
+    # the approximation moves the bandpowers (its size depends on the mask; printed only)
+    rel = np.abs(bp_t - bp_e).max() / np.abs(bp_e).max()
+    print(f"toeplitz vs exact bandpowers rel={rel:.2e}", flush=True)
+    assert np.all(np.isfinite(bp_t)) and rel > 0
+    with pytest.raises(ValueError, match="smaller than"):
+        nmt.NmtWorkspace.from_fields(f0, f2, b, l_toeplitz=47, l_exact=10, dl_band=8)

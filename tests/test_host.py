@@ -58,3 +58,31 @@ def test_nmtbin_errors_and_dell():
     assert np.allclose(f, ell * (ell + 1) / (2 * np.pi))
     e = bins.NmtBin.from_edges([2, 10, 20], [10, 20, 41])
     assert e.get_n_bands() == 3 and e.lmax == 40
+
+
+@pytest.mark.parametrize("lt,le,dl", [(6, 3, 2), (16, 4, 3), (30, 30, 0)])
+def test_toeplitz_pair_list_covers_the_exact_entries(lt, le, dl):
+    from gmaster_amd.workspaces import _toeplitz_pairs
+    from oracle.toeplitz import exact_pairs
+
+    L = 41
+    lo, hi = _toeplitz_pairs(L, lt, le, dl, "cpu")
+    r, c = exact_pairs(L, lt, le, dl)
+    want = np.unique(np.minimum(r, c) * L + np.maximum(r, c))
+    got = lo.numpy().astype(np.int64) * L + hi.numpy()
+    np.testing.assert_array_equal(got, want)
+
+
+def test_toeplitz_sanity_messages():
+    from gmaster_amd.workspaces import toeplitz_sanity
+
+    class Pure:
+        pure_e, pure_b = False, True
+
+    with pytest.raises(ValueError, match="purification"):
+        toeplitz_sanity(6, 3, 2, 12, (Pure(),))
+    with pytest.raises(ValueError, match="must be positive"):
+        toeplitz_sanity(6, 0, 2, 12)
+    with pytest.raises(ValueError, match="smaller than"):
+        toeplitz_sanity(6, 3, 12, 12)
+    toeplitz_sanity(-1, -1, -1, 12)
