@@ -49,6 +49,18 @@ def legendre_flops(nside, lmax):
     return 2 * (2 + 4) * T0, 2 * (2 + 4) * T2
 
 
+def _measured_traffic(nside):
+    """HBM bytes per k_ana_s launch from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE
+    summary at this Nside (copy-kernel scaled), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"k_ana_s_traffic_nside{nside}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def run_pipeline(nmt, mask, tqu, nside, nlb):
     from gmaster_amd import field, sht, workspaces
 
@@ -135,12 +147,13 @@ def main():
     achieved = f2 / (dom_ms * 1e-3)
     roofline = {
         "kernel": "k_ana_s (spin-2 Legendre analysis, gm_lat_ana_s)",
-        "bound": "fp64",
+        "bound": "mfma",
+        "pipe": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
         "achieved": achieved / 1e12,
         "peak": PEAK_FP64 / 1e12,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64,
-        "traffic": None,
+        "traffic": _measured_traffic(nside),
         "flop_per_launch": f2,
         "mean_launch_ms": dom_ms,
     }

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libgmaster_amd.so")
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int
 _p = ctypes.c_void_p
+_d = ctypes.c_double
 
 # name -> argument types (all return int status unless listed in _RET_I64)
 SIGNATURES = {
@@ -36,6 +37,11 @@ SIGNATURES = {
     "gm_coupled_cell": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p],
     "gm_coupling": [_p, _i64, _p, _i64, _p, _p, _p, _p, _p],
     "gm_coupling_pairs": [_p, _i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _p, _p],
+    "gm_mlp_fwd": [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _p, _p, _p],
+    "gm_mlp_bwd": [_p, _p, _i32, _p, _p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p],
+    "gm_to_bf16": [_p, _i64, _p, _p],
+    "gm_adamw": [_p, _p, _p, _p, _p, _i64, _d, _d, _d, _d, _d, _i64, _p],
+    "gm_mse_grad": [_p, _p, _p, _i64, _d, _p, _p, _p],
     "gm_toeplitz": [_p, _i64, _i64, _i64, _i64, _p, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],

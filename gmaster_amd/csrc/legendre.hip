@@ -439,12 +439,7 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
             an[i] = al[M + i + 1];
             gn[i] = ga[M + i + 1];
         }
-        double2 cdn[4], cmn[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            cdn[i] = cf[(int64_t)(M + i) * 2];
-            cmn[i] = cf[(int64_t)(M + i) * 2 + 1];
-        }
+        // (the coefficients are not prefetched here: 16 more scalar doubles spill SGPRs)
         for (int l0 = M; l0 < L; l0 += 4) {
             double a[4], gg[4];
             double2 cdb[4], cmb[4];
@@ -454,10 +449,8 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
                 gg[i] = gn[i];
                 an[i] = al[l0 + 4 + i + 1];
                 gn[i] = ga[l0 + 4 + i + 1];
-                cdb[i] = cdn[i];
-                cmb[i] = cmn[i];
-                cdn[i] = cf[(int64_t)(l0 + 4 + i) * 2];  // next block (zero padded)
-                cmn[i] = cf[(int64_t)(l0 + 4 + i) * 2 + 1];
+                cdb[i] = cf[(int64_t)(l0 + i) * 2];
+                cmb[i] = cf[(int64_t)(l0 + i) * 2 + 1];
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {

@@ -1,0 +1,93 @@
+"""Reduce rocprofv3 CSV output (``--output-format csv``) to short text summaries.
+
+    python -m gmaster_amd.profsum stats <kernel_stats.csv>
+        per-kernel calls / total / average duration
+    python -m gmaster_amd.profsum pmc <counter_collection.csv> COUNTER
+        per-kernel dispatch count, mean and max of one counter
+    python -m gmaster_amd.profsum traffic <fetch.csv> <write.csv> KERNEL COPY_BYTES
+        HBM bytes per launch of KERNEL from FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
+        each scaled by what the same counter reads for the largest device copy kernel
+        dispatch of COPY_BYTES read + COPY_BYTES written in the same run
+"""
+import csv
+import sys
+from collections import defaultdict
+
+_KERNELS = ("k_ana_s0", "k_ana_s", "k_syn_s0", "k_syn_s", "k_coupling_pairs", "k_coupling", "k_toeplitz",
+            "k_ring_fwd_s0", "k_ring_fwd_s", "k_ring_inv_s0", "k_ring_inv_s", "k_fold_s0", "k_fold_s",
+            "k_finish_s0", "k_finish_s", "k_prep_s0", "k_prep_s", "k_lat_tables", "k_cl_partial", "k_cl_final",
+            "k_bin_block", "k_bin_cell")
+
+
+def short(name):
+    for key in _KERNELS:
+        if key + "I" in name or key + "E" in name:
+            tmpl = name.split(key, 1)[1]
+            if tmpl.startswith("I"):
+                return key + "<" + tmpl.split("EEEv")[0].replace("ILi", "").replace("ELi", ",") + ">"
+            return key
+    return name[:60]
+
+
+def stats(path):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print(f"{'kernel':40s} {'calls':>6s} {'total_ms':>10s} {'avg_ms':>10s} {'pct':>6s}")
+    for r in rows:
+        print(f"{short(r['Name']):40s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
+              f"{float(r['AverageNs']) / 1e6:10.3f} {float(r['Percentage']):6.2f}")
+
+
+def _per_dispatch(path, counter):
+    acc = defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r.get("Counter_Name", counter) != counter:
+            continue
+        acc[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return acc
+
+
+def pmc(path, counter):
+    acc = defaultdict(list)
+    for k, v in _per_dispatch(path, counter).items():
+        acc[short(k)] += v
+    print(f"{'kernel':40s} {'dispatches':>10s} {'mean_KiB':>16s} {'max_KiB':>16s}   ({counter})")
+    for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        print(f"{k:40s} {len(v):10d} {sum(v) / len(v):16.1f} {max(v):16.1f}")
+
+
+def _copy_max(acc):
+    vals = [max(v) for k, v in acc.items() if "copy" in k.lower()]
+    if not vals:
+        raise SystemExit("no copy kernel dispatch in the counter file")
+    return max(vals)
+
+
+def traffic(fetch_path, write_path, kernel, copy_bytes):
+    copy_bytes = float(copy_bytes)
+    out = {}
+    for tag, path, counter in (("fetch", fetch_path, "FETCH_SIZE"), ("write", write_path, "WRITE_SIZE")):
+        acc = _per_dispatch(path, counter)
+        ref_kib = _copy_max(acc)
+        scale = copy_bytes / (ref_kib * 1024.0)
+        vals = [x for k, v in acc.items() if short(k).split("<")[0] == kernel for x in v]
+        if not vals:
+            raise SystemExit(f"kernel {kernel} not found in {path}")
+        mean_kib = sum(vals) / len(vals)
+        out[tag] = mean_kib * 1024.0 * scale
+        print(f"{counter}: {kernel} mean {mean_kib:.1f} KiB over {len(vals)} dispatches; copy kernel "
+              f"{ref_kib:.1f} KiB for {copy_bytes / 2**30:.3f} GiB -> scale {scale:.4f}; "
+              f"scaled {out[tag] / 1e9:.3f} GB per launch")
+    print(f"traffic_bytes_per_launch: {out['fetch'] + out['write']:.6e}")
+
+
+if __name__ == "__main__":
+    cmd = sys.argv[1]
+    if cmd == "stats":
+        stats(sys.argv[2])
+    elif cmd == "pmc":
+        pmc(sys.argv[2], sys.argv[3])
+    elif cmd == "traffic":
+        traffic(*sys.argv[2:6])
+    else:
+        raise SystemExit(__doc__)

@@ -181,6 +181,34 @@ int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, const int32
 int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
                 const int32_t* band_start, int64_t nbands, const double* w, double* out, void* stream);
 
+/* ---- learned coupling surrogate: dense network kernels --------------------------------
+ * No counterpart in the reference, whose coupling is exact or Toeplitz only
+ * (gmaster/workspaces.py:542-590, :1141-1194); these train and run the network that predicts
+ * the Toeplitz residual.  bf16 = uint16 bit patterns, round-to-nearest-even. */
+
+/* y = act(x w^T + b): x (batch, n_in) bf16, w (n_out, n_in) bf16, b (n_out) FP32 or NULL;
+ * act 0 = identity, 1 = SiLU.  Writes any of z (pre-activation), y (FP32), y_bf16. */
+int gm_mlp_fwd(const uint16_t* x, const uint16_t* w, const float* b, int64_t batch, int64_t n_in, int64_t n_out,
+               int act, float* z, float* y, uint16_t* y_bf16, void* stream);
+
+/* Backward of gm_mlp_fwd: dz = dy * act'(z) (FP32 and bf16 scratch, batch x n_out),
+ * dw = dz^T x (n_out x n_in), db = column sums of dz, dx = dz w (FP32 and / or bf16, may be NULL). */
+int gm_mlp_bwd(const float* dy, const float* z, int act, const uint16_t* x, const uint16_t* w, int64_t batch,
+               int64_t n_in, int64_t n_out, float* dz, uint16_t* dz_bf16, float* dw, float* db, float* dx,
+               uint16_t* dx_bf16, void* stream);
+
+/* FP32 -> bf16 (round to nearest even). */
+int gm_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream);
+
+/* AdamW step `step` (from 1) on FP32 master weights w, refreshing the bf16 copy if non-NULL. */
+int gm_adamw(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, int64_t n, double lr, double beta1,
+             double beta2, double eps, double weight_decay, int64_t step, void* stream);
+
+/* Weighted squared error: dy = 2 wt (y - t) / n_eff and sq = wt (y - t)^2 (sq may be NULL,
+ * wt may be NULL for weight 1). */
+int gm_mse_grad(const float* y, const float* t, const float* wt, int64_t n, double n_eff, float* dy, float* sq,
+                void* stream);
+
 #ifdef __cplusplus
 }
 #endif
