@@ -42,6 +42,8 @@ SIGNATURES = {
     "gm_to_bf16": [_p, _i64, _p, _p],
     "gm_adamw": [_p, _p, _p, _p, _p, _i64, _d, _d, _d, _d, _d, _i64, _p],
     "gm_mse_grad": [_p, _p, _p, _i64, _d, _p, _p, _p],
+    "gm_coupling_pair_values": [_p, _i64, _p, _i64, _p, _p, _i64, _p, _p],
+    "gm_toeplitz_correct": [_p, _i64, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _p],
     "gm_toeplitz": [_p, _i64, _i64, _i64, _i64, _p, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],

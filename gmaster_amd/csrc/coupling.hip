@@ -135,6 +135,17 @@ __global__ void k_coupling(const double* __restrict__ Wp, int lmax_w, const doub
     store_pair(l1, l2, L, s, m00, m02, m22p, m22m);
 }
 
+// The four row sums (without the column factor) of each listed pair, vals[k][0..3].
+__global__ void k_coupling_pair_values(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g,
+                                       const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+                                       int64_t npairs, double* __restrict__ vals) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npairs) return;
+    double s[4];
+    couple_pair(Wp, lmax_w, g, lo[k], hi[k], s);
+    for (int q = 0; q < 4; ++q) vals[k * 4 + q] = s[q];
+}
+
 // Only the listed pairs (lo[k] <= hi[k]); every other element of the outputs is left as it was.
 __global__ void k_coupling_pairs(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g, int L,
                                  const int32_t* __restrict__ lo, const int32_t* __restrict__ hi, int64_t npairs,
@@ -191,10 +202,81 @@ __global__ void k_toeplitz(const double* __restrict__ M, int n, int lt, int le, 
     out[(int64_t)a * n + b] = v * (2.0 * b + 1.0);
 }
 
+// Which fill of the Toeplitz approximation sets element (i >= j): 0 = an exact value,
+// 1 = the pure Toeplitz extension, 2 = the corner extrapolation from column l_exact (the
+// same precedence as k_toeplitz).
+__device__ __forceinline__ int toeplitz_kind(int i, int j, int n, int lt, int le, int dl) {
+    int kind = 1;
+    if (j <= le) kind = 0;
+    if (i - j <= dl) kind = 0;
+    if (i >= lt && j >= lt) kind = 1;
+    if (i == j) kind = 0;
+    if (i - j >= 1 && i - j <= dl && j <= lt) kind = 0;
+    if (j >= le && j <= lt && i >= n - 1 - lt + j) kind = 2;
+    return kind;
+}
+
+// Learned correction of a Toeplitz-approximated block M (column factor included), in place:
+// for every element not set exactly, with j = min(row, col) and d = |row - col|,
+//     M += e(j, d) sqrt(D_j D_{j+d}) (2 col + 1),   D_k = |M_kk| / (2k + 1),
+// e bilinear in (log j, log d) over the grid lg[0..nl) x dg[0..nd) (increasing integers)
+// and zero outside it.
+__global__ void k_toeplitz_correct(double* __restrict__ M, int n, int lt, int le, int dl,
+                                   const int32_t* __restrict__ lg, int nl, const int32_t* __restrict__ dg, int nd,
+                                   const float* __restrict__ e) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    if (b >= n) return;
+    const int i = a > b ? a : b, j = a > b ? b : a, d = i - j;
+    if (j < lg[0] || j > lg[nl - 1] || d < dg[0] || d > dg[nd - 1]) return;
+    if (toeplitz_kind(i, j, n, lt, le, dl) == 0) return;
+    int p = 0, q = 0;
+    while (p + 2 < nl && lg[p + 1] <= j) ++p;
+    while (q + 2 < nd && dg[q + 1] <= d) ++q;
+    const double tl = nl > 1 ? (log((double)j) - log((double)lg[p])) / (log((double)lg[p + 1]) - log((double)lg[p])) : 0.0;
+    const double td = nd > 1 ? (log((double)d) - log((double)dg[q])) / (log((double)dg[q + 1]) - log((double)dg[q])) : 0.0;
+    const int p1 = nl > 1 ? p + 1 : p, q1 = nd > 1 ? q + 1 : q;
+    const double v = (1.0 - tl) * ((1.0 - td) * e[p * nd + q] + td * e[p * nd + q1]) +
+                     tl * ((1.0 - td) * e[p1 * nd + q] + td * e[p1 * nd + q1]);
+    const double Di = fabs(M[(int64_t)i * n + i]) / (2.0 * i + 1.0);
+    const double Dj = fabs(M[(int64_t)j * n + j]) / (2.0 * j + 1.0);
+    M[(int64_t)a * n + b] += v * sqrt(Di * Dj) * (2.0 * b + 1.0);
+}
+
 }  // namespace
 }  // namespace gm
 
 using namespace gm;
+
+extern "C" int gm_coupling_pair_values(const double* window, int64_t lmax_window, const double* g, int64_t lmax,
+                                       const int32_t* lo, const int32_t* hi, int64_t npairs, double* vals,
+                                       void* stream) {
+    if (!window || !g || !vals || (npairs > 0 && (!lo || !hi)))
+        return fail(GM_E_ARG, "gm_coupling_pair_values: null pointer");
+    if (lmax < 0 || lmax > 65535) return fail(GM_E_ARG, "gm_coupling_pair_values: bad lmax %lld", (long long)lmax);
+    if (lmax_window < 0 || lmax_window > 2 * lmax)
+        return fail(GM_E_ARG, "gm_coupling_pair_values: need 0 <= lmax_window <= 2 lmax");
+    if (npairs < 0) return fail(GM_E_ARG, "gm_coupling_pair_values: negative pair count");
+    if (npairs == 0) return GM_OK;
+    hipLaunchKernelGGL(k_coupling_pair_values, dim3((unsigned)((npairs + 127) / 128)), dim3(128), 0,
+                       (hipStream_t)stream, window, (int)lmax_window, g, lo, hi, npairs, vals);
+    return check_launch("gm_coupling_pair_values");
+}
+
+extern "C" int gm_toeplitz_correct(double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exact, int64_t dl_band,
+                                   const int32_t* l_grid, int64_t n_l, const int32_t* d_grid, int64_t n_d,
+                                   const float* e, void* stream) {
+    if (!M || !l_grid || !d_grid || !e) return fail(GM_E_ARG, "gm_toeplitz_correct: null pointer");
+    if (lmax < 1 || l_toeplitz <= 0 || l_exact <= 0 || dl_band < 0 || l_exact > l_toeplitz || l_toeplitz >= lmax ||
+        dl_band >= lmax)
+        return fail(GM_E_ARG, "gm_toeplitz_correct: need 0 < l_exact <= l_toeplitz < lmax and 0 <= dl_band < lmax");
+    if (n_l < 1 || n_d < 1 || n_l > 4096 || n_d > 4096) return fail(GM_E_ARG, "gm_toeplitz_correct: bad grid sizes");
+    const int n = (int)lmax + 1;
+    hipLaunchKernelGGL(k_toeplitz_correct, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0,
+                       (hipStream_t)stream, M, n, (int)l_toeplitz, (int)l_exact, (int)dl_band, l_grid, (int)n_l,
+                       d_grid, (int)n_d, e);
+    return check_launch("gm_toeplitz_correct");
+}
 
 extern "C" int gm_toeplitz(const double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exact, int64_t dl_band,
                            double* out, void* stream) {

@@ -8,8 +8,11 @@
         HBM bytes per launch of KERNEL from FETCH_SIZE and WRITE_SIZE (KiB per dispatch),
         each scaled by what the same counter reads for the largest device copy kernel
         dispatch of COPY_BYTES read + COPY_BYTES written in the same run
+    python -m gmaster_amd.profsum traffic-summaries <fetch.txt> <write.txt> KERNEL COPY_BYTES
+        the same from two ``pmc`` summaries
 """
 import csv
+import re
 import sys
 from collections import defaultdict
 
@@ -20,12 +23,19 @@ _KERNELS = ("k_ana_s0", "k_ana_s", "k_syn_s0", "k_syn_s", "k_coupling_pairs", "k
 
 
 def short(name):
+    """``k_ana_s<8,4>``-style name of one of this package's kernels (mangled or demangled),
+    else the first 60 characters."""
     for key in _KERNELS:
-        if key + "I" in name or key + "E" in name:
-            tmpl = name.split(key, 1)[1]
-            if tmpl.startswith("I"):
-                return key + "<" + tmpl.split("EEEv")[0].replace("ILi", "").replace("ELi", ",") + ">"
-            return key
+        m = re.search(r"(?<![A-Za-z_])" + key + r"(<[^>]*>|\(|I|E|$)", name)
+        if not m:
+            continue
+        tail = m.group(1)
+        if tail.startswith("<"):
+            return key + tail.replace(" ", "")
+        if tail == "I":
+            tmpl = name[m.end() - 1:]
+            return key + "<" + tmpl[1:].split("EEEv")[0].replace("ILi", "").replace("ELi", ",").replace("Li", "") + ">"
+        return key
     return name[:60]
 
 
@@ -81,6 +91,35 @@ def traffic(fetch_path, write_path, kernel, copy_bytes):
     print(f"traffic_bytes_per_launch: {out['fetch'] + out['write']:.6e}")
 
 
+def _summary_rows(path):
+    rows = []
+    for line in open(path):
+        if line.startswith("#") or line.startswith("kernel") or not line.strip():
+            continue
+        parts = line.rstrip("\n").rsplit(None, 3)
+        rows.append((parts[0].strip(), int(parts[1]), float(parts[2]), float(parts[3])))
+    return rows
+
+
+def traffic_from_summaries(fetch_txt, write_txt, kernel, copy_bytes):
+    """As ``traffic``, from two ``pmc`` summaries (mean per kernel, max for the copy kernel)."""
+    copy_bytes = float(copy_bytes)
+    out = {}
+    for tag, path, counter in (("fetch", fetch_txt, "FETCH_SIZE"), ("write", write_txt, "WRITE_SIZE")):
+        rows = _summary_rows(path)
+        ref_kib = max(r[3] for r in rows if "copy" in r[0].lower() and "rocclr" in r[0].lower())
+        hits = [r for r in rows if short(r[0]).split("<")[0] == kernel]
+        if not hits:
+            raise SystemExit(f"kernel {kernel} not found in {path}")
+        name, n, mean_kib, _ = hits[0]
+        scale = copy_bytes / (ref_kib * 1024.0)
+        out[tag] = mean_kib * 1024.0 * scale
+        print(f"{counter}: {short(name)} mean {mean_kib:.1f} KiB over {n} dispatches; copy kernel "
+              f"{ref_kib:.1f} KiB for {copy_bytes / 2**30:.3f} GiB -> scale {scale:.4f}; "
+              f"scaled {out[tag] / 1e9:.3f} GB per launch")
+    print(f"traffic_bytes_per_launch: {out['fetch'] + out['write']:.6e}")
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1]
     if cmd == "stats":
@@ -89,5 +128,7 @@ if __name__ == "__main__":
         pmc(sys.argv[2], sys.argv[3])
     elif cmd == "traffic":
         traffic(*sys.argv[2:6])
+    elif cmd == "traffic-summaries":
+        traffic_from_summaries(*sys.argv[2:6])
     else:
         raise SystemExit(__doc__)

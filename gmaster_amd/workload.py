@@ -49,22 +49,25 @@ def pixel_vectors(nside, device):
 
 
 def hole_mask(nside, nholes=20, radius_deg=5.0, apo_deg=2.0, kind="C1", seed=1234, bcut_deg=None, device="cuda"):
-    """Apodized mask with ``nholes`` random disc holes (and an optional |b| cut)."""
+    """Apodized mask with ``nholes`` random disc holes (and an optional |b| cut).
+    ``radius_deg`` is one radius or one per hole."""
     rng = np.random.default_rng(seed)
     zc = rng.uniform(-1.0, 1.0, nholes)
     pc = rng.uniform(0.0, 2 * np.pi, nholes)
     sc = np.sqrt(1 - zc * zc)
     centres = torch.as_tensor(np.stack([sc * np.cos(pc), sc * np.sin(pc), zc], 1), device=device)
     vec, _ = pixel_vectors(nside, device)
-    R = math.radians(radius_deg)
+    # one radius for all holes, or one per hole
+    R = torch.as_tensor(np.radians(np.broadcast_to(np.asarray(radius_deg, dtype=np.float64), (nholes,))),
+                        device=device)
     # angle to the nearest disc edge (negative inside a hole), processed in slabs
     npix = vec.shape[0]
     edge = torch.empty(npix, dtype=torch.float64, device=device)
-    slab = 1 << 24
+    slab = max(1, (1 << 27) // max(nholes, 1))
     for s in range(0, npix, slab):
         v = vec[s:s + slab]
         cosang = torch.clamp(v @ centres.T, -1.0, 1.0)
-        edge[s:s + slab] = torch.acos(cosang).min(dim=1).values - R
+        edge[s:s + slab] = (torch.acos(cosang) - R[None, :]).min(dim=1).values
     if bcut_deg is not None:
         lat = torch.asin(torch.clamp(vec[:, 2], -1.0, 1.0))
         edge = torch.minimum(edge, torch.abs(lat) - math.radians(bcut_deg))

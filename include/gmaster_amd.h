@@ -197,6 +197,21 @@ int gm_mlp_bwd(const float* dy, const float* z, int act, const uint16_t* x, cons
                int64_t n_in, int64_t n_out, float* dz, uint16_t* dz_bf16, float* dw, float* db, float* dx,
                uint16_t* dx_bf16, void* stream);
 
+/* The four row sums of gm_coupling (00, 02, 22+, 22-; without the (2 l2 + 1) column factor)
+ * of each pair (lo[k] <= hi[k] <= lmax) into vals[k][4].  Spot checks of an approximate
+ * coupling matrix; no counterpart in the reference. */
+int gm_coupling_pair_values(const double* window, int64_t lmax_window, const double* g, int64_t lmax,
+                            const int32_t* lo, const int32_t* hi, int64_t npairs, double* vals, void* stream);
+
+/* Learned correction of a Toeplitz-approximated block M (column factor included), in place:
+ * every element not set exactly by the fill gets e(j, d) sqrt(D_j D_{j+d}) (2 col + 1), with
+ * j = min(row, col), d = |row - col|, D_k = |M_kk| / (2k + 1), and e (n_l x n_d, FP32)
+ * interpolated bilinearly in (log j, log d) on the strictly increasing int32 grids
+ * l_grid[n_l], d_grid[n_d] (first entries >= 1), zero outside them. */
+int gm_toeplitz_correct(double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exact, int64_t dl_band,
+                        const int32_t* l_grid, int64_t n_l, const int32_t* d_grid, int64_t n_d, const float* e,
+                        void* stream);
+
 /* FP32 -> bf16 (round to nearest even). */
 int gm_to_bf16(const float* x, int64_t n, uint16_t* y, void* stream);
 

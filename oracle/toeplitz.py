@@ -75,3 +75,55 @@ def apply_to_mcm_block(M, l_toeplitz, l_exact, dl_band):
     """Toeplitz-approximate a coupling block ``M`` (column factor (2 l' + 1) included)."""
     col = (2 * np.arange(len(M)) + 1)[None, :]
     return apply_toeplitz(M / col, l_toeplitz, l_exact, dl_band) * col
+
+
+def entry_kind(n, l_toeplitz, l_exact, dl_band):
+    """``(n, n)`` int array: 0 where the fill keeps the exact value, 1 where it writes the pure
+    Toeplitz extension, 2 where it writes the corner extrapolation from column l_exact
+    (read off the fill sequence of ``apply_toeplitz``)."""
+    lt, le, dl = l_toeplitz, l_exact, dl_band
+    kind = np.ones((n, n), dtype=np.int64)
+    kind[:, : le + 1] = 0
+    for off in range(dl + 1):
+        i = np.arange(n - off)
+        kind[i + off, i] = 0
+    kind[lt:, lt:] = 1
+    i = np.arange(n)
+    kind[i, i] = 0
+    for off in range(1, dl + 1):
+        cnt = min(n - off, lt + 1)
+        i = np.arange(cnt)
+        kind[i + off, i] = 0
+    for off in range(lt - le + 1):
+        col = off + le
+        kind[n - 1 - lt + col:, col] = 2
+    lower = np.tril(kind)
+    return lower + np.tril(kind, -1).T
+
+
+def correct(M, l_toeplitz, l_exact, dl_band, l_grid, d_grid, e):
+    """Learned correction of a Toeplitz block ``M`` (column factor included): every
+    non-exact element gets ``e(j, d) sqrt(D_j D_{j+d}) (2 col + 1)``, e bilinear in
+    (log j, log d) on the grids, zero outside them."""
+    M = np.array(M, dtype=np.float64)
+    n = len(M)
+    kind = entry_kind(n, l_toeplitz, l_exact, dl_band)
+    lg = np.asarray(l_grid, np.float64)
+    dg = np.asarray(d_grid, np.float64)
+    e = np.asarray(e, np.float64).reshape(len(lg), len(dg))
+    D = np.abs(np.diag(M)) / (2 * np.arange(n) + 1)
+    out = M.copy()
+    for a in range(n):
+        for b in range(n):
+            i, j = max(a, b), min(a, b)
+            d = i - j
+            if kind[a, b] == 0 or j < lg[0] or j > lg[-1] or d < dg[0] or d > dg[-1]:
+                continue
+            p = min(np.searchsorted(lg, j, side="right") - 1, len(lg) - 2) if len(lg) > 1 else 0
+            q = min(np.searchsorted(dg, d, side="right") - 1, len(dg) - 2) if len(dg) > 1 else 0
+            p1, q1 = min(p + 1, len(lg) - 1), min(q + 1, len(dg) - 1)
+            tl = (np.log(j) - np.log(lg[p])) / (np.log(lg[p1]) - np.log(lg[p])) if p1 != p else 0.0
+            td = (np.log(d) - np.log(dg[q])) / (np.log(dg[q1]) - np.log(dg[q])) if q1 != q else 0.0
+            v = (1 - tl) * ((1 - td) * e[p, q] + td * e[p, q1]) + tl * ((1 - td) * e[p1, q] + td * e[p1, q1])
+            out[a, b] += v * np.sqrt(D[i] * D[j]) * (2 * b + 1)
+    return out

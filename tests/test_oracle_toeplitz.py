@@ -136,3 +136,54 @@ def test_sanity_messages(args, msg):
     with pytest.raises(ValueError, match=msg):
         sanity(*args)
     sanity(-1, -1, -1, 12)
+
+
+@pytest.mark.parametrize("lt,le,dl", PARAMS)
+def test_entry_kind_marks_exactly_the_kept_entries(lt, le, dl):
+    from oracle.toeplitz import entry_kind
+
+    S = _random_symmetric(40, 21)
+    out = apply_toeplitz(S, lt, le, dl)
+    kind = entry_kind(40, lt, le, dl)
+    assert np.array_equal(kind, kind.T)
+    # kind 0 entries are the input's values (column l_exact rebuilt by the corner fill aside)
+    np.testing.assert_array_equal(out[kind == 0], S[kind == 0])
+    # the per-element rule agrees on which fill is last
+    for a in range(40):
+        for b in range(40):
+            i, j = max(a, b), min(a, b)
+            k = 1
+            if j <= le or i - j <= dl:
+                k = 0
+            if i >= lt and j >= lt:
+                k = 1
+            if i == j or (1 <= i - j <= dl and j <= lt):
+                k = 0
+            if le <= j <= lt and i >= 40 - 1 - lt + j:
+                k = 2
+            assert kind[a, b] == k
+
+
+def test_correction_with_zero_residual_is_identity_and_is_bilinear():
+    from oracle.toeplitz import correct, entry_kind
+
+    S = _random_symmetric(40, 4) + 60 * np.eye(40)
+    col = 2 * np.arange(40) + 1
+    M = S * col[None]
+    lg, dg = np.array([3, 9, 30]), np.array([1, 4, 16])
+    np.testing.assert_array_equal(correct(M, 20, 2, 1, lg, dg, np.zeros(9)), M)
+    # a constant e adds e sqrt(D_i D_j) (2 col + 1) on every covered non-exact element
+    out = correct(M, 20, 2, 1, lg, dg, np.full(9, 0.5))
+    kind = entry_kind(40, 20, 2, 1)
+    D = np.abs(np.diag(S))
+    i, j = np.meshgrid(np.arange(40), np.arange(40), indexing="ij")
+    lo, d = np.minimum(i, j), np.abs(i - j)
+    cov = (kind != 0) & (lo >= 3) & (lo <= 30) & (d >= 1) & (d <= 16)
+    want = M + np.where(cov, 0.5 * np.sqrt(D[i] * D[j]) * col[None], 0.0)
+    np.testing.assert_allclose(out, want, rtol=1e-15)
+    # bilinear in logs: on a grid node the node value is used
+    e = np.arange(9, dtype=float)
+    out = correct(M, 20, 2, 1, lg, dg, e)
+    a, b = 9 + 4, 9  # j = 9 (node 1), d = 4 (node 1) -> e[1, 1] = 4
+    assert kind[a, b] != 0
+    assert out[a, b] - M[a, b] == pytest.approx(4.0 * np.sqrt(D[a] * D[b]) * col[b], rel=1e-14)
