@@ -12,6 +12,12 @@ Workload (synthetic, seeded): mask = 1 except 20 random 5-degree disc holes, C1-
 2 degrees; T, Q, U Gaussian with C_TT = 400/(l+10)^2, C_EE = 0.1 C_TT, C_TE = 0.3
 sqrt(TT EE), C_BB = 0.01 C_EE; lmax = 3 nside - 1; bins of 16.
 
+Two forms are timed in the same run: plain FP64 (exact 3j coupling) and the chosen form
+(``--form``, default ``fast``: the coupling from the Toeplitz approximation with a wide exact
+region, judged by exact spot checks; every transform stays FP64).  ``value`` is the chosen
+form's time; ``fp64_s`` the plain form's, and ``bandpower_err_vs_fp64`` the chosen form's
+largest relative bandpower error per spectrum.
+
 Prints one JSON line (value = seconds per pipeline, lower is better) with a roofline
 entry for the dominant kernel (spin-2 Legendre analysis, FP64 VALU bound) measured with
 HIP events on the launching stream during the timed steps, and a CPU baseline (the
@@ -61,9 +67,10 @@ def _measured_traffic(nside):
     return None
 
 
-def run_pipeline(nmt, mask, tqu, nside, nlb):
+def run_pipeline(nmt, mask, tqu, nside, nlb, form="fp64", info=None):
     from gmaster_amd import field, sht, workspaces
 
+    nmt.set_form(form)
     sht.clear_plans()
     workspaces.clear_coupling_cache()
     field.clear_mask_cache()
@@ -74,6 +81,34 @@ def run_pipeline(nmt, mask, tqu, nside, nlb):
     for f1, f2 in ((fT, fT), (fT, fP), (fP, fP)):
         w = nmt.NmtWorkspace.from_fields(f1, f2, b)
         out.append(w.decouple_cell(nmt.compute_coupled_cell(f1, f2)))
+        if info is not None:
+            info.append(w.coupling_info)
+    nmt.set_form("fp64")
+    return out
+
+
+def bandpower_errors(bp, ref):
+    """Relative error per spectrum of decoupled bandpowers against the FP64 form: auto and TE
+    spectra relative to themselves, the null spectra (TB, EB, BE) relative to
+    sqrt(XX YY) of the band."""
+    import torch
+
+    tt, te, tb = ref[0][0], ref[1][0], ref[1][1]
+    ee, eb, be, bb = ref[2]
+    out = {}
+
+    def rel(a, r, norm=None):
+        norm = torch.abs(r) if norm is None else norm
+        return float(torch.max(torch.abs(a - r) / norm))
+
+    out["TT"] = rel(bp[0][0], tt)
+    out["TE"] = rel(bp[1][0], te)
+    out["EE"] = rel(bp[2][0], ee)
+    out["BB"] = rel(bp[2][3], bb)
+    out["TB"] = rel(bp[1][1], tb, torch.sqrt(torch.abs(tt * bb)))
+    out["EB"] = rel(bp[2][1], eb, torch.sqrt(torch.abs(ee * bb)))
+    out["BE"] = rel(bp[2][2], be, torch.sqrt(torch.abs(ee * bb)))
+    out["max"] = max(out.values())
     return out
 
 
@@ -86,6 +121,8 @@ def main():
     ap.add_argument("--nlb", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU sample time")
+    ap.add_argument("--form", default="fast", choices=["fp64", "fast"],
+                    help="form timed for the metric (fp64 is always timed beside it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scales PMC byte counters)")
     args = ap.parse_args()
@@ -118,29 +155,42 @@ def main():
         del src, dst
         log("calibration copy: 1 GiB read + 1 GiB written")
 
-    for i in range(args.warmup):
-        t = time.time()
-        run_pipeline(nmt, mask, tqu, nside, nlb)
-        torch.cuda.synchronize()
-        log(f"warmup {i}: {time.time() - t:.3f}s")
+    def timed(form, with_timers):
+        for i in range(args.warmup):
+            t = time.time()
+            run_pipeline(nmt, mask, tqu, nside, nlb, form)
+            torch.cuda.synchronize()
+            log(f"{form} warmup {i}: {time.time() - t:.3f}s")
+        timers = _lib.enable_timers(True) if with_timers else None
+        times, info = [], []
+        bp = None
+        for i in range(args.steps):
+            info.clear()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            bp = run_pipeline(nmt, mask, tqu, nside, nlb, form, info)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+            log(f"{form} step {i}: {times[-1]:.3f}s")
+        if with_timers:
+            _lib.enable_timers(False)
+        return times, bp, timers, list(info)
 
-    timers = _lib.enable_timers(True)
-    times = []
-    for i in range(args.steps):
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        bp = run_pipeline(nmt, mask, tqu, nside, nlb)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t)
-        log(f"step {i}: {times[-1]:.3f}s")
-    _lib.enable_timers(False)
+    forms = ["fp64"] if args.form == "fp64" else ["fp64", args.form]
+    runs = {f: timed(f, f == "fp64") for f in forms}
+    times_fp64, bp_fp64, timers, _ = runs["fp64"]
+    times, bp, _, coupling_info = runs[args.form]
     total = sum(times)
+    bp_err = bandpower_errors(bp, bp_fp64) if args.form != "fp64" else None
+    if bp_err is not None:
+        log(f"{args.form} vs fp64 bandpower errors: {bp_err}")
 
     # per entry point: mean duration per launch (ms) over the timed steps
     stages = {}
     for name, evs in timers.items():
         ms = [s.elapsed_time(e) for s, e in evs]
         stages[name] = {"calls": len(ms), "mean_ms": sum(ms) / len(ms), "total_ms_per_step": sum(ms) / args.steps}
+    fp64_s = sum(times_fp64) / len(times_fp64)
     f0, f2 = legendre_flops(nside, lmax)
     dom = "gm_lat_ana_s"
     dom_ms = stages.get(dom, {}).get("mean_ms", float("nan"))
@@ -183,7 +233,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "form": "fp64",
+        "form": args.form,
+        "fp64_s": fp64_s,
+        "speedup_vs_fp64": fp64_s / (total / args.steps),
+        "bandpower_err_vs_fp64": bp_err,
+        "coupling": coupling_info[-1] if coupling_info else None,
         "config": {"workload": f"T+P full MASTER, HEALPix Nside {nside}, lmax {lmax}, n_iter 3, "
                                f"20 x 5deg holes C1 2deg, bins of {nlb}", "nside": nside, "lmax": lmax,
                    "n_iter": 3, "nlb": nlb, "parallelism": "single GPU"},
@@ -194,6 +248,8 @@ def main():
             "per_call": stages,
         },
         "bandpowers_finite": bool(all(torch.isfinite(x).all() for x in bp)),
+        "timed_steps_s": times,
+        "fp64_steps_s": times_fp64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -9,7 +9,8 @@ Every transform, coupling matrix and spectrum runs in the HIP library
 
 from . import _lib, sht  # noqa: F401
 from ._array import GmArray
-from ._config import get_default_params, nmt_params, set_n_iter_default, set_sht_calculator, set_tol_pinv_default
+from ._config import (get_default_params, nmt_params, set_form, set_n_iter_default, set_sht_calculator,
+                      set_tol_pinv_default)
 from .bins import NmtBin
 from .field import NmtField
 from .utils import (
@@ -38,6 +39,7 @@ __all__ = [
     "map2alm_pair",
     "mask_apodization",
     "nmt_params",
+    "set_form",
     "set_n_iter_default",
     "set_sht_calculator",
     "set_tol_pinv_default",

@@ -9,6 +9,9 @@ class NmtParams:
         self.n_iter_default = 3
         self.n_iter_mask_default = 3
         self.tol_pinv_default = 1e-10
+        self.form = "fp64"
+        self.fast_toeplitz = (2000, 256, 256)
+        self.fast_tol = 1e-3
 
 
 nmt_params = NmtParams()
@@ -35,10 +38,36 @@ def set_sht_calculator(calculator):
     nmt_params.sht_calculator = str(calculator)
 
 
+FORMS = ("fp64", "fast")
+
+
+def set_form(form, toeplitz=None, tol=None):
+    """Select how workspaces build the coupling matrix.
+
+    * ``"fp64"`` (default): the exact Wigner-3j coupling of every entry.
+    * ``"fast"``: the Toeplitz approximation with a wide exact region (``toeplitz`` =
+      ``(l_toeplitz, l_exact, dl_band)`` in physical multipoles, default (2000, 256, 256)),
+      judged by exact spot checks of random non-exact entries: when their largest error in
+      correlation units exceeds ``tol`` (default 1e-3), the exact coupling is used instead.
+      Below lmax = 2 l_toeplitz the exact coupling is used (it is cheap there).
+    """
+    if form not in FORMS:
+        raise ValueError(f"form must be one of {FORMS}")
+    nmt_params.form = form
+    if toeplitz is not None:
+        lt, le, dl = (int(v) for v in toeplitz)
+        if not (0 < le <= lt and dl >= 0):
+            raise ValueError("need 0 < l_exact <= l_toeplitz and dl_band >= 0")
+        nmt_params.fast_toeplitz = (lt, le, dl)
+    if tol is not None:
+        nmt_params.fast_tol = float(tol)
+
+
 def get_default_params():
     return {
         "sht_calculator": nmt_params.sht_calculator,
         "n_iter": nmt_params.n_iter_default,
         "n_iter_mask": nmt_params.n_iter_mask_default,
         "tol_pinv": nmt_params.tol_pinv_default,
+        "form": nmt_params.form,
     }

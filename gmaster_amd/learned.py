@@ -111,7 +111,9 @@ def residual_targets(exact, approx, lmax, grid):
     dg = torch.as_tensor(grid.d_grid, dtype=torch.long, device=dev)
     J = lg[:, None].expand(len(lg), len(dg))
     I = J + dg[None, :]
-    ok = I <= lmax
+    ok = I <= lmax  # implies J <= lmax
+    # grid points beyond this lmax get weight 0; their indices are clamped into the matrix
+    Jc = torch.clamp(J, max=lmax)
     Ic = torch.clamp(I, max=lmax)
     col = (2 * torch.arange(lmax + 1, dtype=torch.float64, device=dev) + 1)
     t = []
@@ -119,8 +121,8 @@ def residual_targets(exact, approx, lmax, grid):
         S = exact[k] / col[None, :]
         ST = approx[k] / col[None, :]
         D = torch.abs(torch.diagonal(S))
-        den = torch.sqrt(D[J] * D[Ic])
-        e = torch.where(ok & (den > 0), (S[J, Ic] - ST[J, Ic]) / torch.where(den > 0, den, 1.0), 0.0)
+        den = torch.sqrt(D[Jc] * D[Ic])
+        e = torch.where(ok & (den > 0), (S[Jc, Ic] - ST[Jc, Ic]) / torch.where(den > 0, den, 1.0), 0.0)
         t.append(e)
     targets = torch.stack(t).reshape(-1).to(torch.float32)
     weights = ok[None].expand(len(BLOCKS), -1, -1).reshape(-1).to(torch.float32)
@@ -241,7 +243,7 @@ class Surrogate:
         toe = coupling_blocks(window, lmax, g.toeplitz())
         blocks = {k: v.clone() for k, v in toe.items()}
         self.correct(blocks, window, lmax)
-        err_t, err_l = _spot_check(window, lmax, g, toe, blocks, n_check, seed)
+        err_t, err_l = _spot_check(window, lmax, g.toeplitz(), toe, blocks, n_check, seed)
         info = {"fallback": False, "sample_err_toeplitz": err_t, "sample_err_learned": err_l, "n_check": n_check}
         if tol is not None and err_l > tol:
             info["fallback"] = True
@@ -278,7 +280,13 @@ class Surrogate:
                    t["y_scale"].to(dev), json.loads(md.get("meta", "{}")))
 
 
-def _spot_check(window, lmax, grid, toe, learned, n_check, seed):
+def spot_check(window, lmax, toeplitz, blocks, n_check=4096, seed=0):
+    """Max |error| in correlation units of approximate blocks over random entries outside
+    the exact columns and band of ``toeplitz``, against exact 3j values."""
+    return _spot_check(window, lmax, toeplitz, blocks, blocks, n_check, seed)[1]
+
+
+def _spot_check(window, lmax, toeplitz, toe, learned, n_check, seed):
     """Max |error| in correlation units of the Toeplitz and learned blocks over random
     non-exact upper-triangle entries, against the exact 3j values."""
     from .workspaces import _g_table
@@ -286,7 +294,7 @@ def _spot_check(window, lmax, grid, toe, learned, n_check, seed):
     dev = toe["00"].device
     L = lmax + 1
     rng = np.random.default_rng(seed)
-    lt, le, dl = grid.toeplitz()
+    lt, le, dl = toeplitz
     lo = rng.integers(le + 1, L - 1, 4 * n_check)
     hi = lo + rng.integers(1, L, 4 * n_check)
     keep = (hi < L) & (hi - lo > dl)

@@ -21,6 +21,7 @@ import torch
 
 from . import _lib
 from ._array import device_tensor, wrap
+from ._config import nmt_params
 
 _COUPLING_CACHE = {}
 _G_CACHE = {}
@@ -110,6 +111,26 @@ def coupling_blocks(window, lmax, toeplitz=None):
         _COUPLING_CACHE.clear()
     _COUPLING_CACHE[key] = blocks
     return blocks
+
+
+def fast_coupling_blocks(window, lmax, toeplitz=None, tol=None, n_check=4096, seed=0):
+    """The ``"fast"`` form: Toeplitz approximation with a wide exact region (only the exact
+    entries it reads are evaluated), judged by exact values of ``n_check`` random non-exact
+    entries; falls back to the exact coupling when their largest error in correlation units
+    exceeds ``tol``.  Returns ``(blocks, info)``."""
+    from .learned import spot_check
+
+    lt, le, dl = toeplitz or nmt_params.fast_toeplitz
+    tol = nmt_params.fast_tol if tol is None else tol
+    if lmax < 2 * lt:
+        return coupling_blocks(window, lmax), {"form": "fast", "fallback": True, "reason": "lmax < 2 l_toeplitz"}
+    toe = coupling_blocks(window, lmax, (lt, le, dl))
+    err = spot_check(window, lmax, (lt, le, dl), toe, n_check, seed)
+    info = {"form": "fast", "toeplitz": (lt, le, dl), "sample_err": err, "n_check": n_check, "tol": tol,
+            "fallback": bool(err > tol)}
+    if err > tol:
+        return coupling_blocks(window, lmax), info
+    return toe, info
 
 
 def clear_coupling_cache():
@@ -207,7 +228,11 @@ class NmtWorkspace:
             npix = fl1.minfo.npix
             self.wawb = float(torch.sum(fl1.mask * fl2.mask)) * (4 * np.pi / npix) / (4 * np.pi)
         self.toeplitz = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
-        self.blocks = coupling_blocks(self.pcl_mask, self.lmax, self.toeplitz)
+        self.coupling_info = {"form": "toeplitz" if self.toeplitz else nmt_params.form}
+        if self.toeplitz is None and nmt_params.form == "fast":
+            self.blocks, self.coupling_info = fast_coupling_blocks(self.pcl_mask, self.lmax)
+        else:
+            self.blocks = coupling_blocks(self.pcl_mask, self.lmax, self.toeplitz)
         self.layout = _layout(self.ncls, self.spin1, self.spin2)
         self._postprocess()
 

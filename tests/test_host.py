@@ -86,3 +86,30 @@ def test_toeplitz_sanity_messages():
     with pytest.raises(ValueError, match="smaller than"):
         toeplitz_sanity(6, 3, 12, 12)
     toeplitz_sanity(-1, -1, -1, 12)
+
+
+def test_residual_targets_handle_lmax_below_the_grid():
+    """Grid multipoles above lmax get weight 0 and are never used as indices (CPU tensors
+    raise on an out-of-range index, as the device does)."""
+    import torch
+
+    from gmaster_amd import learned
+
+    g = learned.ResidualGrid()
+    lmax = 767
+    rng = np.random.default_rng(0)
+    ex = {k: torch.as_tensor(rng.uniform(1, 2, (lmax + 1, lmax + 1))) for k in learned.BLOCKS}
+    ap = {k: v * 0.5 for k, v in ex.items()}
+    t, w = learned.residual_targets(ex, ap, lmax, g)
+    assert t.shape == (g.n_out,) and w.shape == (g.n_out,)
+    J = np.repeat(g.l_grid, len(g.d_grid))
+    I = J + np.tile(g.d_grid, len(g.l_grid))
+    np.testing.assert_array_equal(w.numpy().reshape(4, -1)[0], (I <= lmax).astype(np.float32))
+    assert bool(torch.all(t[w == 0] == 0))
+    # S - S_T = S / 2 here, so e = 0.5 S_ij / sqrt(D_i D_j) at valid points
+    col = 2 * np.arange(lmax + 1) + 1
+    S = ex["00"].numpy() / col[None, :]
+    D = np.abs(np.diag(S))
+    ok = I <= lmax
+    want = 0.5 * S[J[ok], I[ok]] / np.sqrt(D[J[ok]] * D[I[ok]])
+    np.testing.assert_allclose(t.numpy()[: len(J)][ok], want, rtol=1e-6)
