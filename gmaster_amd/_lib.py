@@ -27,13 +27,17 @@ SIGNATURES = {
     "gm_ring_fold_residual_c": [_p, _p, _p, _i64, _i64, _i64, _p],
     "gm_lat_tables": [_i64, _i64, _p, _p, _p, _p],
     "gm_lat_ana_s0": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
+    "gm_lat_ana_s0_mixed": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_finish_s0": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
     "gm_lat_prep_s0": [_p, _i64, _i64, _p, _i64, _p, _p],
     "gm_lat_syn_s0": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "gm_lat_syn_s0_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
     "gm_lat_ana_s": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
+    "gm_lat_ana_s_mixed": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_finish_s": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _p],
     "gm_lat_prep_s": [_p, _p, _p, _i64, _i64, _p, _p],
     "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
+    "gm_lat_syn_s_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_coupled_cell": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p],
     "gm_coupling": [_p, _i64, _p, _i64, _p, _p, _p, _p, _p],
     "gm_coupling_pairs": [_p, _i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _p, _p],

@@ -23,26 +23,46 @@ namespace gm {
 
 namespace {
 
-constexpr double kInactiveMax = 0x1p-300;
-constexpr double kRescale = 0x1p-600;
+// Extended-exponent constants per arithmetic type: a lane whose seed lies below 2^thr
+// starts with k < 0 and mu = lambda 2^(unit |k|) < 2^inactive; it is rescaled by 2^-unit
+// whenever |mu| passes 2^inactive.  FP32 keeps inactive lanes below 2^-36, far under its
+// 2^-24 resolution, and active seeds above 2^-100, inside its normal range.
+template <typename F> struct ExtExp;
+template <> struct ExtExp<double> {
+    static constexpr double inactive_max = 0x1p-300, rescale = 0x1p-600;
+    static constexpr int thr = -900, unit = 600;
+};
+template <> struct ExtExp<float> {
+    static constexpr float inactive_max = 0x1p-36f, rescale = 0x1p-64f;
+    static constexpr int thr = -100, unit = 64;
+};
+
+template <typename F> struct cpx { F x, y; };
+
+template <typename F>
+__device__ inline cpx<F> load_c(const double2* p) {
+    const double2 v = *p;
+    return cpx<F>{(F)v.x, (F)v.y};
+}
 
 // lambda_M for order m at a ring as (mu, k): mant * 2^e from the host constant and
-// the powers sin^(m+s)(theta/2) cos^|m-s|(theta/2).
-__device__ inline void seed_lane(double smant, int sexp, int a, int b, double shalf, double chalf,
-                                 double& mu, int& k) {
+// the powers sin^(m+s)(theta/2) cos^|m-s|(theta/2), evaluated in FP64.
+template <typename F>
+__device__ inline void seed_lane(double smant, int sexp, int a, int b, double shalf, double chalf, F& mu, int& k) {
     double m1, m2;
     int e1, e2;
     pow_ext(shalf, a, m1, e1);
     pow_ext(chalf, b, m2, e2);
     const double v = smant * m1 * m2;
     const int E = sexp + e1 + e2;
-    if (E >= -900) {
+    constexpr int thr = ExtExp<F>::thr, unit = ExtExp<F>::unit;
+    if (E >= thr) {
         k = 0;
-        mu = ldexp(v, E);
+        mu = (F)ldexp(v, E);
     } else {
-        const int kk = (-900 - E + 599) / 600;
+        const int kk = (thr - E + unit - 1) / unit;
         k = -kk;
-        mu = ldexp(v, E + 600 * kk);
+        mu = (F)ldexp(v, E + unit * kk);
     }
 }
 
@@ -50,36 +70,36 @@ __device__ inline void seed_lane(double smant, int sexp, int a, int b, double sh
 // acc[NV] per thread -> out[0..NV) = sum over all NW*64 threads; one barrier per call
 // (double-buffered LDS: `buf` alternates between calls).  Threads o*TPO .. (o+1)*TPO-1
 // sum output o, TPO = 64*NW / NV, each over NV thread slots strided by TPO (bank-conflict
-// free), then combine with TPO-lane shuffles.
-template <int NV, int NW>
-__device__ inline void block_reduce(const double (&acc)[NV], double* lds, int buf, double* out) {
+// free), then combine with TPO-lane shuffles.  The result is stored in FP64.
+template <int NV, int NW, typename F>
+__device__ inline void block_reduce(const F (&acc)[NV], F* lds, int buf, double* out) {
     constexpr int T = 64 * NW;
     constexpr int TPO = T / NV;
-    double* base = lds + buf * (NV * T);
+    F* base = lds + buf * (NV * T);
     const int tid = threadIdx.x;
 #pragma unroll
     for (int v = 0; v < NV; ++v) base[v * T + tid] = acc[v];
     __syncthreads();
     const int o = tid / TPO, sl = tid % TPO;
     // thread slot sl sums lanes sl, sl + TPO, ... : consecutive threads read consecutive words
-    const double* src = base + o * T + sl;
-    double s = 0.0;
+    const F* src = base + o * T + sl;
+    F s = 0;
 #pragma unroll
     for (int u = 0; u < NV; ++u) s += src[u * TPO];
 #pragma unroll
     for (int msk = 1; msk < TPO; msk <<= 1) s += __shfl_xor(s, msk);
-    if (sl == 0) out[o] = s;
+    if (sl == 0) out[o] = (double)s;
 }
 
-template <int R>
-__device__ inline void rescale(double (&mu0)[R], double (&mu1)[R], int (&kx)[R], bool& anyinact) {
+template <int R, typename F>
+__device__ inline void rescale(F (&mu0)[R], F (&mu1)[R], int (&kx)[R], bool& anyinact) {
     if (!anyinact) return;
     bool still = false;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
-        if (kx[t] < 0 && fabs(mu0[t]) > kInactiveMax) {
-            mu0[t] *= kRescale;
-            mu1[t] *= kRescale;
+        if (kx[t] < 0 && fabs(mu0[t]) > ExtExp<F>::inactive_max) {
+            mu0[t] *= ExtExp<F>::rescale;
+            mu1[t] *= ExtExp<F>::rescale;
             kx[t] += 1;
         }
         still = still || (kx[t] < 0);
@@ -89,7 +109,10 @@ __device__ inline void rescale(double (&mu0)[R], double (&mu1)[R], int (&kx)[R],
 
 // ------------------------------------------------------------------ spin-0 analysis
 // Folded lanes: j = 0 is the equator ring, j > 0 the pair (north r = 2n-1-j, south 2n-1+j).
-template <int K, int R, int NW>
+// The recurrence always runs in FP64; F is the arithmetic type of the contraction: each
+// step's mu is rounded to F once and multiplied into F accumulators (spectra are read as
+// FP64 and converted, partial sums stored in FP64).
+template <int K, int R, int NW, typename F>
 __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ spec, int64_t ld,
                                                     int64_t smap, int nside, int L,
                                                     const double* __restrict__ alpha,
@@ -100,14 +123,14 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
     constexpr int NV = 4 * 2 * K;  // 4 degrees x (re, im) x maps
     constexpr int T = 64 * NW;
     constexpr int CH = T * R;
-    __shared__ double lds[2 * NV * T];
+    __shared__ F lds[2 * NV * T];
     const int c = blockIdx.x, m = blockIdx.y;
     const int na = nact[m];
     if (c * CH >= na) return;
 
     double x[R], mu0[R], mu1[R];
     int kx[R];
-    double2 gE[R][K], gO[R][K];
+    cpx<F> gE[R][K], gO[R][K];
     const double sm = smant[m];
     const int se = sexp[m];
 #pragma unroll
@@ -118,25 +141,25 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
             const int r = 2 * nside - 1 - j, rp = 2 * nside - 1 + j;
             const Ring g = ring_info(nside, r);
             x[t] = g.z;
-            seed_lane(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
+            seed_lane<double>(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const double2 a = spec[k * smap + (int64_t)r * ld + m];
                 if (j == 0) {
-                    gE[t][k] = a;
-                    gO[t][k] = make_double2(0.0, 0.0);
+                    gE[t][k] = cpx<F>{(F)a.x, (F)a.y};
+                    gO[t][k] = cpx<F>{0, 0};
                 } else {
                     const double2 b = spec[k * smap + (int64_t)rp * ld + m];
-                    gE[t][k] = cadd(a, b);
-                    gO[t][k] = csub(a, b);
+                    gE[t][k] = cpx<F>{(F)(a.x + b.x), (F)(a.y + b.y)};
+                    gO[t][k] = cpx<F>{(F)(a.x - b.x), (F)(a.y - b.y)};
                 }
             }
         } else {
-            x[t] = 0.0;
-            mu0[t] = 0.0;
+            x[t] = 0;
+            mu0[t] = 0;
             kx[t] = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) gE[t][k] = gO[t][k] = make_double2(0.0, 0.0);
+            for (int k = 0; k < K; ++k) gE[t][k] = gO[t][k] = cpx<F>{0, 0};
         }
     }
     bool anyinact = false;
@@ -159,18 +182,18 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
             a[i] = an[i];
             an[i] = al[l0 + 4 + i + 1];  // prefetch the next block's coefficients
         }
-        double acc[NV];
+        F acc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        for (int v = 0; v < NV; ++v) acc[v] = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             // l0 + m is even, so degree l0 + i pairs with the even (E) combination for even i
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                const double cur = (i & 1) ? mu1[t] : mu0[t];
+                const F cur = (F)((i & 1) ? mu1[t] : mu0[t]);
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    const double2 gv = (i & 1) ? gO[t][k] : gE[t][k];
+                    const cpx<F> gv = (i & 1) ? gO[t][k] : gE[t][k];
                     acc[i * 2 * K + 2 * k] = fma(cur, gv.x, acc[i * 2 * K + 2 * k]);
                     acc[i * 2 * K + 2 * k + 1] = fma(cur, gv.y, acc[i * 2 * K + 2 * k + 1]);
                 }
@@ -181,13 +204,13 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
                 else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
             }
         }
-        rescale<R>(mu0, mu1, kx, anyinact);
-        block_reduce<NV, NW>(acc, lds, buf, part + (int64_t)l0 * (2 * K));
+        rescale<R, double>(mu0, mu1, kx, anyinact);
+        block_reduce<NV, NW, F>(acc, lds, buf, part + (int64_t)l0 * (2 * K));
     }
 }
 
 // ------------------------------------------------------------------ spin-0 synthesis
-template <int K, int R, int NW>
+template <int K, int R, int NW, typename F>
 __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ coef, int nside, int L,
                                                     const double* __restrict__ alpha,
                                                     const double* __restrict__ smant,
@@ -202,25 +225,25 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
 
     double x[R], mu0[R], mu1[R];
     int kx[R];
-    double2 aE[R][K], aO[R][K];
+    cpx<F> aE[R][K], aO[R][K];
     const double sm = smant[m];
     const int se = sexp[m];
     bool anyact = false;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         const int j = c * CH + t * T + threadIdx.x;
-        mu1[t] = 0.0;
+        mu1[t] = 0;
         kx[t] = 0;
-        x[t] = 0.0;
-        mu0[t] = 0.0;
+        x[t] = 0;
+        mu0[t] = 0;
         if (j < na) {
             const Ring g = ring_info(nside, 2 * nside - 1 - j);
             x[t] = g.z;
-            seed_lane(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
+            seed_lane<double>(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
             anyact = true;
         }
 #pragma unroll
-        for (int k = 0; k < K; ++k) aE[t][k] = aO[t][k] = make_double2(0.0, 0.0);
+        for (int k = 0; k < K; ++k) aE[t][k] = aO[t][k] = cpx<F>{0, 0};
     }
     if (__any(anyact)) {
         bool anyinact = false;
@@ -232,14 +255,14 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
         double an[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) an[i] = al[m + i + 1];
-        double2 cn[4][K];
+        cpx<F> cn[4][K];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int k = 0; k < K; ++k) cn[i][k] = cf[(int64_t)(m + i) * K + k];
+            for (int k = 0; k < K; ++k) cn[i][k] = load_c<F>(cf + (int64_t)(m + i) * K + k);
         for (int l0 = m; l0 < L; l0 += 4) {
             double a[4];
-            double2 cb[4][K];
+            cpx<F> cb[4][K];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 a[i] = an[i];
@@ -247,20 +270,20 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     cb[i][k] = cn[i][k];
-                    cn[i][k] = cf[(int64_t)(l0 + 4 + i) * K + k];  // next block (zero padded)
+                    cn[i][k] = load_c<F>(cf + (int64_t)(l0 + 4 + i) * K + k);  // next block (zero padded)
                 }
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                double2 cv[K];
+                cpx<F> cv[K];
 #pragma unroll
                 for (int k = 0; k < K; ++k) cv[k] = cb[i][k];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    const double cur = (i & 1) ? mu1[t] : mu0[t];
+                    const F cur = (F)((i & 1) ? mu1[t] : mu0[t]);
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
-                        double2& acc = (i & 1) ? aO[t][k] : aE[t][k];
+                        cpx<F>& acc = (i & 1) ? aO[t][k] : aE[t][k];
                         acc.x = fma(cur, cv[k].x, acc.x);
                         acc.y = fma(cur, cv[k].y, acc.y);
                     }
@@ -271,7 +294,7 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
                     else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
                 }
             }
-            rescale<R>(mu0, mu1, kx, anyinact);
+            rescale<R, double>(mu0, mu1, kx, anyinact);
         }
     }
 #pragma unroll
@@ -282,8 +305,9 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             // F_N = E + O, F_S = E - O (zero for rings beyond the polar-skip bound)
-            spec[k * smap + (int64_t)r * ld + m] = cadd(aE[t][k], aO[t][k]);
-            if (j > 0) spec[k * smap + (int64_t)rp * ld + m] = csub(aE[t][k], aO[t][k]);
+            const double ex = aE[t][k].x, ey = aE[t][k].y, ox = aO[t][k].x, oy = aO[t][k].y;
+            spec[k * smap + (int64_t)r * ld + m] = make_double2(ex + ox, ey + oy);
+            if (j > 0) spec[k * smap + (int64_t)rp * ld + m] = make_double2(ex - ox, ey - oy);
         }
     }
 }
@@ -298,7 +322,7 @@ __device__ inline int slot_ring(int nside, int j) {
     return (j & 1) ? 2 * nside - 1 - u : 2 * nside - 1 + u;
 }
 
-template <int R, int NW>
+template <int R, int NW, typename F>
 __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ spec, int64_t ld, int nside,
                                                    int L, int s, const double* __restrict__ alpha,
                                                    const double* __restrict__ gamma,
@@ -309,7 +333,7 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
     constexpr int NV = 16;  // 4 degrees x 4 channels
     constexpr int T = 64 * NW;
     constexpr int CH = T * R;
-    __shared__ double lds[2 * NV * T];
+    __shared__ F lds[2 * NV * T];
     const int c = blockIdx.x, m = blockIdx.y;
     const int na = nact[m];
     if (c * CH >= na) return;
@@ -318,7 +342,7 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
 
     double x[R], mu0[R], mu1[R];
     int kx[R];
-    double2 gD[R], gM[R];
+    cpx<F> gD[R], gM[R];
     const double sm = smant[m];
     const int se = sexp[m];
     const int a_exp = m + s, b_exp = m > s ? m - s : s - m;
@@ -330,14 +354,14 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
             const int r = slot_ring(nside, j);
             const Ring g = ring_info(nside, r);
             x[t] = g.z;
-            seed_lane(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
-            gD[t] = spec[(int64_t)r * ld + (L - 1) + m];
-            gM[t] = m ? spec[(int64_t)(nring - 1 - r) * ld + (L - 1) - m] : make_double2(0.0, 0.0);
+            seed_lane<double>(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
+            gD[t] = load_c<F>(spec + (int64_t)r * ld + (L - 1) + m);
+            gM[t] = m ? load_c<F>(spec + (int64_t)(nring - 1 - r) * ld + (L - 1) - m) : cpx<F>{0, 0};
         } else {
-            x[t] = 0.0;
-            mu0[t] = 0.0;
+            x[t] = 0;
+            mu0[t] = 0;
             kx[t] = 0;
-            gD[t] = gM[t] = make_double2(0.0, 0.0);
+            gD[t] = gM[t] = cpx<F>{0, 0};
         }
     }
     bool anyinact = false;
@@ -364,14 +388,14 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
             an[i] = al[l0 + 4 + i + 1];
             gn[i] = ga[l0 + 4 + i + 1];
         }
-        double acc[NV];
+        F acc[NV];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+        for (int v = 0; v < NV; ++v) acc[v] = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
 #pragma unroll
             for (int t = 0; t < R; ++t) {
-                const double cur = (i & 1) ? mu1[t] : mu0[t];
+                const F cur = (F)((i & 1) ? mu1[t] : mu0[t]);
                 acc[4 * i + 0] = fma(cur, gD[t].x, acc[4 * i + 0]);
                 acc[4 * i + 1] = fma(cur, gD[t].y, acc[4 * i + 1]);
                 acc[4 * i + 2] = fma(cur, gM[t].x, acc[4 * i + 2]);
@@ -383,12 +407,12 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
                 else mu1[t] = fma(fma(x[t], a[i], gg[i]), mu0[t], -mu1[t]);
             }
         }
-        rescale<R>(mu0, mu1, kx, anyinact);
-        block_reduce<NV, NW>(acc, lds, buf, part + (int64_t)l0 * 4);
+        rescale<R, double>(mu0, mu1, kx, anyinact);
+        block_reduce<NV, NW, F>(acc, lds, buf, part + (int64_t)l0 * 4);
     }
 }
 
-template <int R, int NW>
+template <int R, int NW, typename F>
 __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ coef, int nside, int L, int s,
                                                    const double* __restrict__ alpha,
                                                    const double* __restrict__ gamma,
@@ -405,7 +429,7 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
 
     double x[R], mu0[R], mu1[R];
     int kx[R];
-    double2 aD[R], aM[R];
+    cpx<F> aD[R], aM[R];
     const double sm = smant[m];
     const int se = sexp[m];
     const int a_exp = m + s, b_exp = m > s ? m - s : s - m;
@@ -413,15 +437,15 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
 #pragma unroll
     for (int t = 0; t < R; ++t) {
         const int j = c * CH + t * T + threadIdx.x;
-        mu1[t] = 0.0;
+        mu1[t] = 0;
         kx[t] = 0;
-        x[t] = 0.0;
-        mu0[t] = 0.0;
-        aD[t] = aM[t] = make_double2(0.0, 0.0);
+        x[t] = 0;
+        mu0[t] = 0;
+        aD[t] = aM[t] = cpx<F>{0, 0};
         if (j < na && M < L) {
             const Ring g = ring_info(nside, slot_ring(nside, j));
             x[t] = g.z;
-            seed_lane(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
+            seed_lane<double>(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
             anyact = true;
         }
     }
@@ -442,22 +466,22 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
         // (the coefficients are not prefetched here: 16 more scalar doubles spill SGPRs)
         for (int l0 = M; l0 < L; l0 += 4) {
             double a[4], gg[4];
-            double2 cdb[4], cmb[4];
+            cpx<F> cdb[4], cmb[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 a[i] = an[i];
                 gg[i] = gn[i];
                 an[i] = al[l0 + 4 + i + 1];
                 gn[i] = ga[l0 + 4 + i + 1];
-                cdb[i] = cf[(int64_t)(l0 + i) * 2];
-                cmb[i] = cf[(int64_t)(l0 + i) * 2 + 1];
+                cdb[i] = load_c<F>(cf + (int64_t)(l0 + i) * 2);
+                cmb[i] = load_c<F>(cf + (int64_t)(l0 + i) * 2 + 1);
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const double2 cd = cdb[i], cm = cmb[i];
+                const cpx<F> cd = cdb[i], cm = cmb[i];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
-                    const double cur = (i & 1) ? mu1[t] : mu0[t];
+                    const F cur = (F)((i & 1) ? mu1[t] : mu0[t]);
                     aD[t].x = fma(cur, cd.x, aD[t].x);
                     aD[t].y = fma(cur, cd.y, aD[t].y);
                     aM[t].x = fma(cur, cm.x, aM[t].x);
@@ -469,7 +493,7 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
                     else mu1[t] = fma(fma(x[t], a[i], gg[i]), mu0[t], -mu1[t]);
                 }
             }
-            rescale<R>(mu0, mu1, kx, anyinact);
+            rescale<R, double>(mu0, mu1, kx, anyinact);
         }
     }
 #pragma unroll
@@ -477,8 +501,8 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
         const int j = c * CH + t * T + threadIdx.x;
         if (j >= nring) continue;
         const int r = slot_ring(nside, j);
-        spec[(int64_t)r * ld + (L - 1) + m] = aD[t];
-        if (m) spec[(int64_t)(nring - 1 - r) * ld + (L - 1) - m] = aM[t];
+        spec[(int64_t)r * ld + (L - 1) + m] = make_double2(aD[t].x, aD[t].y);
+        if (m) spec[(int64_t)(nring - 1 - r) * ld + (L - 1) - m] = make_double2(aM[t].x, aM[t].y);
     }
 }
 
@@ -669,33 +693,132 @@ extern "C" int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* 
 }
 
 
-extern "C" int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
-                             int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
-                             const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+namespace gm {
+namespace {
+
+template <typename F>
+int launch_ana_s0(const char* name, const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
+                  int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
+                  const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
     if (int e = check_lat(nside, lmax, 0)) return e;
-    if (!spec || !alpha || !seed_mant || !seed_exp || !nactive || !partial)
-        return fail(GM_E_ARG, "gm_lat_ana_s0: null pointer");
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_ana_s0: nmaps must be 1 or 2");
+    if (!spec || !alpha || !seed_mant || !seed_exp || !nactive || !partial) return fail(GM_E_ARG, "%s: null pointer", name);
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "%s: nmaps must be 1 or 2", name);
     const int64_t L = lmax + 1;
-    if (ld < L) return fail(GM_E_SHAPE, "gm_lat_ana_s0: ld < lmax + 1");
-    if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "gm_lat_ana_s0: map stride too small");
+    if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
+    if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
     const LatConfig cfg = lat_config(nside, 0);
     if (nchunks != cfg.nchunks)
-        return fail(GM_E_SHAPE, "gm_lat_ana_s0: partial has %lld chunks, need %d", (long long)nchunks, cfg.nchunks);
+        return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
     const int64_t ts = table_size(L);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* sp = reinterpret_cast<const double2*>(spec);
+#define GM_ANA0(KK, RR, WW) hipLaunchKernelGGL((k_ana_s0<KK, RR, WW, F>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts)
     if (nmaps == 1) {
-        if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s0<1, 2, 1>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
-        else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s0<1, 4, 2>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
-        else hipLaunchKernelGGL((k_ana_s0<1, 6, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        if (cfg.R == 2) GM_ANA0(1, 2, 1);
+        else if (cfg.R == 4) GM_ANA0(1, 4, 2);
+        else GM_ANA0(1, 6, 4);
     } else {
-        if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s0<2, 2, 1>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
-        else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s0<2, 4, 2>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
-        else hipLaunchKernelGGL((k_ana_s0<2, 6, 4>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts);
+        if (cfg.R == 2) GM_ANA0(2, 2, 1);
+        else if (cfg.R == 4) GM_ANA0(2, 4, 2);
+        else GM_ANA0(2, 6, 4);
     }
-    return check_launch("gm_lat_ana_s0");
+#undef GM_ANA0
+    return check_launch(name);
+}
+
+template <typename F>
+int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
+                  const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, double* spec, int64_t ld,
+                  int64_t smap, void* stream) {
+    if (int e = check_lat(nside, lmax, 0)) return e;
+    if (!coef || !alpha || !seed_mant || !seed_exp || !nactive || !spec) return fail(GM_E_ARG, "%s: null pointer", name);
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "%s: nmaps must be 1 or 2", name);
+    const int64_t L = lmax + 1;
+    if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
+    if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
+    const LatConfig cfg = syn_config(nside, 0);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* cf = reinterpret_cast<const double2*>(coef);
+    double2* sp = reinterpret_cast<double2*>(spec);
+#define GM_SYN0(KK, RR, WW) hipLaunchKernelGGL((k_syn_s0<KK, RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap)
+    if (nmaps == 1) {
+        if (cfg.R == 2) GM_SYN0(1, 2, 1);
+        else if (cfg.R == 4) GM_SYN0(1, 4, 2);
+        else GM_SYN0(1, 6, 4);
+    } else {
+        if (cfg.R == 2) GM_SYN0(2, 2, 1);
+        else if (cfg.R == 4) GM_SYN0(2, 4, 2);
+        else GM_SYN0(2, 6, 4);
+    }
+#undef GM_SYN0
+    return check_launch(name);
+}
+
+template <typename F>
+int launch_ana_s(const char* name, const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
+                 const double* alpha, const double* gamma, const double* seed_mant, const int32_t* seed_exp,
+                 const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+    if (int e = check_lat(nside, lmax, spin)) return e;
+    if (spin == 0) return fail(GM_E_ARG, "%s: use the spin-0 entry for spin 0", name);
+    if (!spec || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !partial)
+        return fail(GM_E_ARG, "%s: null pointer", name);
+    const int64_t L = lmax + 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "%s: ld < 2 lmax + 1", name);
+    const LatConfig cfg = lat_config(nside, spin);
+    if (nchunks != cfg.nchunks)
+        return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* sp = reinterpret_cast<const double2*>(spec);
+    const int64_t ts = table_size(L);
+#define GM_ANAS(RR, WW) hipLaunchKernelGGL((k_ana_s<RR, WW, F>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts)
+    if (cfg.R == 2) GM_ANAS(2, 1);
+    else if (cfg.R == 4) GM_ANAS(4, 2);
+    else GM_ANAS(8, 4);
+#undef GM_ANAS
+    return check_launch(name);
+}
+
+template <typename F>
+int launch_syn_s(const char* name, const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
+                 const double* gamma, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+                 double* spec, int64_t ld, void* stream) {
+    if (int e = check_lat(nside, lmax, spin)) return e;
+    if (spin == 0) return fail(GM_E_ARG, "%s: use the spin-0 entry for spin 0", name);
+    if (!coef || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !spec)
+        return fail(GM_E_ARG, "%s: null pointer", name);
+    const int64_t L = lmax + 1;
+    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "%s: ld < 2 lmax + 1", name);
+    const LatConfig cfg = syn_config(nside, spin);
+    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    hipStream_t st = (hipStream_t)stream;
+    const double2* cf = reinterpret_cast<const double2*>(coef);
+    double2* sp = reinterpret_cast<double2*>(spec);
+#define GM_SYNS(RR, WW) hipLaunchKernelGGL((k_syn_s<RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld)
+    if (cfg.R == 2) GM_SYNS(2, 1);
+    else if (cfg.R == 4) GM_SYNS(4, 2);
+    else GM_SYNS(8, 4);
+#undef GM_SYNS
+    return check_launch(name);
+}
+
+}  // namespace
+}  // namespace gm
+
+extern "C" int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
+                             int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
+                             const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+    return launch_ana_s0<double>("gm_lat_ana_s0", spec, nmaps, ld, smap, nside, lmax, alpha, seed_mant, seed_exp,
+                                 nactive, partial, nchunks, stream);
+}
+
+extern "C" int gm_lat_ana_s0_mixed(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
+                                   int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
+                                   const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+    return launch_ana_s0<float>("gm_lat_ana_s0_mixed", spec, nmaps, ld, smap, nside, lmax, alpha, seed_mant, seed_exp,
+                                nactive, partial, nchunks, stream);
 }
 
 extern "C" int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
@@ -727,50 +850,30 @@ extern "C" int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_
 extern "C" int gm_lat_syn_s0(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
                              const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, double* spec,
                              int64_t ld, int64_t smap, void* stream) {
-    if (int e = check_lat(nside, lmax, 0)) return e;
-    if (!coef || !alpha || !seed_mant || !seed_exp || !nactive || !spec)
-        return fail(GM_E_ARG, "gm_lat_syn_s0: null pointer");
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_syn_s0: nmaps must be 1 or 2");
-    const int64_t L = lmax + 1;
-    if (ld < L) return fail(GM_E_SHAPE, "gm_lat_syn_s0: ld < lmax + 1");
-    if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "gm_lat_syn_s0: map stride too small");
-    const LatConfig cfg = syn_config(nside, 0);
-    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
-    hipStream_t st = (hipStream_t)stream;
-    const double2* cf = reinterpret_cast<const double2*>(coef);
-    double2* sp = reinterpret_cast<double2*>(spec);
-    if (nmaps == 1) {
-        if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s0<1, 2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-        else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s0<1, 4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-        else hipLaunchKernelGGL((k_syn_s0<1, 6, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-    } else {
-        if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s0<2, 2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-        else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s0<2, 4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-        else hipLaunchKernelGGL((k_syn_s0<2, 6, 4>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap);
-    }
-    return check_launch("gm_lat_syn_s0");
+    return launch_syn_s0<double>("gm_lat_syn_s0", coef, nmaps, nside, lmax, alpha, seed_mant, seed_exp, nactive, spec,
+                                 ld, smap, stream);
+}
+
+extern "C" int gm_lat_syn_s0_mixed(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
+                                   const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+                                   double* spec, int64_t ld, int64_t smap, void* stream) {
+    return launch_syn_s0<float>("gm_lat_syn_s0_mixed", coef, nmaps, nside, lmax, alpha, seed_mant, seed_exp, nactive,
+                                spec, ld, smap, stream);
 }
 
 extern "C" int gm_lat_ana_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
                             const double* alpha, const double* gamma, const double* seed_mant, const int32_t* seed_exp,
                             const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
-    if (int e = check_lat(nside, lmax, spin)) return e;
-    if (spin == 0) return fail(GM_E_ARG, "gm_lat_ana_s: use gm_lat_ana_s0 for spin 0");
-    if (!spec || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !partial)
-        return fail(GM_E_ARG, "gm_lat_ana_s: null pointer");
-    const int64_t L = lmax + 1;
-    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_lat_ana_s: ld < 2 lmax + 1");
-    const LatConfig cfg = lat_config(nside, spin);
-    if (nchunks != cfg.nchunks)
-        return fail(GM_E_SHAPE, "gm_lat_ana_s: partial has %lld chunks, need %d", (long long)nchunks, cfg.nchunks);
-    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
-    hipStream_t st = (hipStream_t)stream;
-    const double2* sp = reinterpret_cast<const double2*>(spec);
-    const int64_t ts = table_size(L);
-    if (cfg.R == 2) hipLaunchKernelGGL((k_ana_s<2, 1>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts);
-    else if (cfg.R == 4) hipLaunchKernelGGL((k_ana_s<4, 2>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts);
-    else hipLaunchKernelGGL((k_ana_s<8, 4>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts);
-    return check_launch("gm_lat_ana_s");
+    return launch_ana_s<double>("gm_lat_ana_s", spec, ld, nside, lmax, spin, alpha, gamma, seed_mant, seed_exp,
+                                nactive, partial, nchunks, stream);
+}
+
+extern "C" int gm_lat_ana_s_mixed(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
+                                  const double* alpha, const double* gamma, const double* seed_mant,
+                                  const int32_t* seed_exp, const int32_t* nactive, double* partial, int64_t nchunks,
+                                  void* stream) {
+    return launch_ana_s<float>("gm_lat_ana_s_mixed", spec, ld, nside, lmax, spin, alpha, gamma, seed_mant, seed_exp,
+                               nactive, partial, nchunks, stream);
 }
 
 extern "C" int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
@@ -800,19 +903,13 @@ extern "C" int gm_lat_prep_s(const double* alm_e, const double* alm_b, const dou
 extern "C" int gm_lat_syn_s(const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
                             const double* gamma, const double* seed_mant, const int32_t* seed_exp,
                             const int32_t* nactive, double* spec, int64_t ld, void* stream) {
-    if (int e = check_lat(nside, lmax, spin)) return e;
-    if (spin == 0) return fail(GM_E_ARG, "gm_lat_syn_s: use gm_lat_syn_s0 for spin 0");
-    if (!coef || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !spec)
-        return fail(GM_E_ARG, "gm_lat_syn_s: null pointer");
-    const int64_t L = lmax + 1;
-    if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_lat_syn_s: ld < 2 lmax + 1");
-    const LatConfig cfg = syn_config(nside, spin);
-    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
-    hipStream_t st = (hipStream_t)stream;
-    const double2* cf = reinterpret_cast<const double2*>(coef);
-    double2* sp = reinterpret_cast<double2*>(spec);
-    if (cfg.R == 2) hipLaunchKernelGGL((k_syn_s<2, 1>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);
-    else if (cfg.R == 4) hipLaunchKernelGGL((k_syn_s<4, 2>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);
-    else hipLaunchKernelGGL((k_syn_s<8, 4>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);
-    return check_launch("gm_lat_syn_s");
+    return launch_syn_s<double>("gm_lat_syn_s", coef, nside, lmax, spin, alpha, gamma, seed_mant, seed_exp, nactive,
+                                spec, ld, stream);
+}
+
+extern "C" int gm_lat_syn_s_mixed(const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
+                                  const double* gamma, const double* seed_mant, const int32_t* seed_exp,
+                                  const int32_t* nactive, double* spec, int64_t ld, void* stream) {
+    return launch_syn_s<float>("gm_lat_syn_s_mixed", coef, nside, lmax, spin, alpha, gamma, seed_mant, seed_exp,
+                               nactive, spec, ld, stream);
 }

@@ -21,7 +21,14 @@ _MASK_ALMS = {}
 
 
 def _mask_key(mask, lmax_mask, n_iter_mask):
-    return (mask.data_ptr(), mask._version, mask.numel(), str(mask.device), int(lmax_mask), int(n_iter_mask))
+    return (mask.data_ptr(), mask._version, mask.numel(), str(mask.device), int(lmax_mask), int(n_iter_mask),
+            _precision())
+
+
+def _precision():
+    """Transform precision of the selected form: ``"fast"`` contracts in FP32 over an FP64
+    recurrence (``sht._refine``), ``"fp64"`` keeps every pass in FP64."""
+    return "mixed" if nmt_params.form == "fast" else "fp64"
 
 
 def clear_mask_cache():
@@ -102,12 +109,13 @@ class NmtField:
         self.alm_mask = _MASK_ALMS.get(key)
         if (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
                 and self.n_iter == self.n_iter_mask and self.alm_mask is None):
-            pair = sht.map2alm_s0(torch.cat([masked, self.mask[None, :]]), lmax, n_iter=self.n_iter)
+            pair = sht.map2alm_s0(torch.cat([masked, self.mask[None, :]]), lmax, n_iter=self.n_iter,
+                                  precision=_precision())
             self.alm = pair[0:1]
             self.alm_mask = pair[1]
             _remember(key, self.alm_mask)
         else:
-            self.alm = sht.map2alm(masked, self.spin, lmax, n_iter=self.n_iter)
+            self.alm = sht.map2alm(masked, self.spin, lmax, n_iter=self.n_iter, precision=_precision())
         if not self.lite:
             self.maps = masked
 
@@ -134,7 +142,8 @@ class NmtField:
             key = _mask_key(self.mask, self.ainfo_mask.lmax, self.n_iter_mask)
             alm = _MASK_ALMS.get(key)
             if alm is None:
-                alm = sht.map2alm_s0(self.mask[None, :], self.ainfo_mask.lmax, n_iter=self.n_iter_mask)[0]
+                alm = sht.map2alm_s0(self.mask[None, :], self.ainfo_mask.lmax, n_iter=self.n_iter_mask,
+                                      precision=_precision())[0]
                 _remember(key, alm)
             if self.lite:
                 return wrap(alm)

@@ -148,28 +148,69 @@ def _as_f64(t, device):
 
 
 # ------------------------------------------------------------------------------ spin 0
-def _analysis_s0(p, spec, alm, mode):
+def _analysis_s0(p, spec, alm, mode, mixed=False):
     K = spec.shape[0]
     st = _lib.stream_ptr(p.device)
     partial = torch.empty(p.nchunks * p.tsize * 2 * K, dtype=torch.float64, device=p.device)
     ld = spec.shape[2]
-    _lib.call("gm_lat_ana_s0", _lib.ptr(spec), K, ld, spec.stride(0), p.nside, p.lmax, _lib.ptr(p.alpha),
+    name, alpha = ("gm_lat_ana_s0_mixed" if mixed else "gm_lat_ana_s0"), p.alpha
+    _lib.call(name, _lib.ptr(spec), K, ld, spec.stride(0), p.nside, p.lmax, _lib.ptr(alpha),
               _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), p.nchunks, st)
     _lib.call("gm_lat_finish_s0", _lib.ptr(partial), K, p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax,
               _lib.ptr(alm), alm.stride(0), int(mode), st)
 
 
-def _synthesis_s0(p, alm, spec):
+def _synthesis_s0(p, alm, spec, mixed=False):
     K = alm.shape[0]
     st = _lib.stream_ptr(p.device)
     coef = torch.empty(p.tsize * K, dtype=torch.complex128, device=p.device)
     _lib.call("gm_lat_prep_s0", _lib.ptr(alm), K, alm.stride(0), _lib.ptr(p.P), p.lmax, _lib.ptr(coef), st)
-    _lib.call("gm_lat_syn_s0", _lib.ptr(coef), K, p.nside, p.lmax, _lib.ptr(p.alpha), _lib.ptr(p.smant),
+    name, alpha = ("gm_lat_syn_s0_mixed" if mixed else "gm_lat_syn_s0"), p.alpha
+    _lib.call(name, _lib.ptr(coef), K, p.nside, p.lmax, _lib.ptr(alpha), _lib.ptr(p.smant),
               _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(spec), spec.shape[2], spec.stride(0), st)
 
 
-def map2alm_s0(maps, lmax, n_iter=0, polar_skip=True):
-    """``(K, npix)`` real maps (K = 1 or 2, sharing one recurrence) -> ``(K, nelem)`` alms."""
+def _refine(alm, n_iter, analysis, synthesis, fold, spec, mixed):
+    """Jacobi refinement ``alm <- alm - A(S alm - map)`` in ring-Fourier space
+    (``gmaster/_sht/healpix.py:1459-1472``).
+
+    FP64 (``mixed`` False): every pass recomputes the full residual from ``alm``.
+    Mixed (FP32 contraction, FP64 recurrence): after R_0 = fold(S alm_0) - G the passes act
+    on corrections only, D = A R, alm -= D, R <- R - fold(S D), so the rounding of each
+    pass scales with the correction, not with the signal.  The residual buffer holds s R
+    (s = +-1, flipped by each fold)."""
+    if n_iter <= 0:
+        return alm
+    synth = torch.empty_like(spec)
+    if not mixed:
+        resid = torch.empty_like(spec)
+        for _ in range(int(n_iter)):
+            synthesis(alm, synth, False)
+            fold(synth, spec, resid)
+            analysis(resid, alm, 1, False)
+        return alm
+    ra, rb = torch.empty_like(spec), torch.empty_like(spec)
+    synthesis(alm, synth, True)
+    fold(synth, spec, ra)
+    sign = 1
+    d = torch.empty_like(alm)
+    for it in range(int(n_iter)):
+        analysis(ra, d, 0, True)
+        if sign > 0:
+            alm.sub_(d)
+        else:
+            alm.add_(d)
+        if it < n_iter - 1:
+            synthesis(d, synth, True)
+            fold(synth, ra, rb)
+            ra, rb = rb, ra
+            sign = -sign
+    return alm
+
+
+def map2alm_s0(maps, lmax, n_iter=0, polar_skip=True, precision="fp64"):
+    """``(K, npix)`` real maps (K = 1 or 2, sharing one recurrence) -> ``(K, nelem)`` alms.
+    ``precision="mixed"`` runs every pass with FP32 contraction (FP64 recurrence)."""
     maps = torch.as_tensor(maps)
     dev = maps.device if maps.is_cuda else torch.device("cuda")
     maps = _as_f64(maps, dev)
@@ -186,16 +227,20 @@ def map2alm_s0(maps, lmax, n_iter=0, polar_skip=True):
     _lib.call("gm_ring_fft_fwd_s0", _lib.ptr(maps), K, npix, nside, lmax, _lib.ptr(spec), L, spec.stride(0), st)
     nelem = L * (L + 1) // 2
     alm = torch.empty((K, nelem), dtype=torch.complex128, device=dev)
-    _analysis_s0(p, spec, alm, 0)
-    if n_iter > 0:
-        synth = torch.empty_like(spec)
-        resid = torch.empty_like(spec)
-        for _ in range(int(n_iter)):
-            _synthesis_s0(p, alm, synth)
-            _lib.call("gm_ring_fold_residual", _lib.ptr(synth), _lib.ptr(spec), _lib.ptr(resid), K, L,
-                      spec.stride(0), nside, lmax, st)
-            _analysis_s0(p, resid, alm, 1)
-    return alm
+    _analysis_s0(p, spec, alm, 0, _mixed(precision))
+
+    def fold(f, g, out):
+        _lib.call("gm_ring_fold_residual", _lib.ptr(f), _lib.ptr(g), _lib.ptr(out), K, L, spec.stride(0), nside, lmax,
+                  st)
+
+    return _refine(alm, n_iter, lambda s_, a, mode, mx: _analysis_s0(p, s_, a, mode, mx),
+                   lambda a, s_, mx: _synthesis_s0(p, a, s_, mx), fold, spec, _mixed(precision))
+
+
+def _mixed(precision):
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    return precision == "mixed"
 
 
 def alm2map_s0(alm, nside, lmax, polar_skip=True):
@@ -219,26 +264,28 @@ def alm2map_s0(alm, nside, lmax, polar_skip=True):
 
 
 # ------------------------------------------------------------------------------ spin s
-def _analysis_s(p, spec, ae, ab, mode):
+def _analysis_s(p, spec, ae, ab, mode, mixed=False):
     st = _lib.stream_ptr(p.device)
     partial = torch.empty(p.nchunks * p.tsize * 4, dtype=torch.float64, device=p.device)
-    _lib.call("gm_lat_ana_s", _lib.ptr(spec), spec.shape[1], p.nside, p.lmax, p.spin, _lib.ptr(p.alpha),
-              _lib.ptr(p.gamma), _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial),
+    name, (alpha, gamma) = ("gm_lat_ana_s_mixed" if mixed else "gm_lat_ana_s"), (p.alpha, p.gamma)
+    _lib.call(name, _lib.ptr(spec), spec.shape[1], p.nside, p.lmax, p.spin, _lib.ptr(alpha),
+              _lib.ptr(gamma), _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial),
               p.nchunks, st)
     _lib.call("gm_lat_finish_s", _lib.ptr(partial), p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax, p.spin,
               _lib.ptr(ae), _lib.ptr(ab), int(mode), st)
 
 
-def _synthesis_s(p, ae, ab, spec):
+def _synthesis_s(p, ae, ab, spec, mixed=False):
     st = _lib.stream_ptr(p.device)
     coef = torch.empty(p.tsize * 2, dtype=torch.complex128, device=p.device)
     _lib.call("gm_lat_prep_s", _lib.ptr(ae), _lib.ptr(ab), _lib.ptr(p.P), p.lmax, p.spin, _lib.ptr(coef), st)
-    _lib.call("gm_lat_syn_s", _lib.ptr(coef), p.nside, p.lmax, p.spin, _lib.ptr(p.alpha), _lib.ptr(p.gamma),
+    name, (alpha, gamma) = ("gm_lat_syn_s_mixed" if mixed else "gm_lat_syn_s"), (p.alpha, p.gamma)
+    _lib.call(name, _lib.ptr(coef), p.nside, p.lmax, p.spin, _lib.ptr(alpha), _lib.ptr(gamma),
               _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(spec), spec.shape[1], st)
 
 
-def map2alm_spin(maps, lmax, spin, n_iter=0, polar_skip=True):
-    """``(2, npix)`` (Q, U) -> ``(2, nelem)`` (E, B)."""
+def map2alm_spin(maps, lmax, spin, n_iter=0, polar_skip=True, precision="fp64"):
+    """``(2, npix)`` (Q, U) -> ``(2, nelem)`` (E, B).  ``precision="mixed"`` as for spin 0."""
     maps = torch.as_tensor(maps)
     dev = maps.device if maps.is_cuda else torch.device("cuda")
     maps = _as_f64(maps, dev)
@@ -256,15 +303,13 @@ def map2alm_spin(maps, lmax, spin, n_iter=0, polar_skip=True):
     _lib.call("gm_ring_fft_fwd_s", _lib.ptr(maps[0]), _lib.ptr(maps[1]), nside, lmax, _lib.ptr(spec), W, st)
     nelem = L * (L + 1) // 2
     alm = torch.empty((2, nelem), dtype=torch.complex128, device=dev)
-    _analysis_s(p, spec, alm[0], alm[1], 0)
-    if n_iter > 0:
-        synth = torch.empty_like(spec)
-        resid = torch.empty_like(spec)
-        for _ in range(int(n_iter)):
-            _synthesis_s(p, alm[0], alm[1], synth)
-            _lib.call("gm_ring_fold_residual_c", _lib.ptr(synth), _lib.ptr(spec), _lib.ptr(resid), W, nside, lmax, st)
-            _analysis_s(p, resid, alm[0], alm[1], 1)
-    return alm
+    _analysis_s(p, spec, alm[0], alm[1], 0, _mixed(precision))
+
+    def fold(f, g, out):
+        _lib.call("gm_ring_fold_residual_c", _lib.ptr(f), _lib.ptr(g), _lib.ptr(out), W, nside, lmax, st)
+
+    return _refine(alm, n_iter, lambda s_, a, mode, mx: _analysis_s(p, s_, a[0], a[1], mode, mx),
+                   lambda a, s_, mx: _synthesis_s(p, a[0], a[1], s_, mx), fold, spec, _mixed(precision))
 
 
 def alm2map_spin(alm, nside, lmax, spin, polar_skip=True):
@@ -287,10 +332,10 @@ def alm2map_spin(alm, nside, lmax, spin, polar_skip=True):
     return maps
 
 
-def map2alm(maps, spin, lmax, n_iter=0, polar_skip=True):
+def map2alm(maps, spin, lmax, n_iter=0, polar_skip=True, precision="fp64"):
     if int(spin) == 0:
-        return map2alm_s0(maps, lmax, n_iter, polar_skip)
-    return map2alm_spin(maps, lmax, spin, n_iter, polar_skip)
+        return map2alm_s0(maps, lmax, n_iter, polar_skip, precision)
+    return map2alm_spin(maps, lmax, spin, n_iter, polar_skip, precision)
 
 
 def alm2map(alm, spin, nside, lmax, polar_skip=True):

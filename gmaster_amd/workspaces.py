@@ -53,21 +53,34 @@ def toeplitz_sanity(l_toeplitz, l_exact, dl_band, lmax, fields=()):
         raise ValueError("`l_toeplitz`, `l_exact` and `dl_band` must be smaller than `l_max`")
 
 
+def _runs(starts, counts, device):
+    """Concatenated integer runs ``starts[i] + (0 .. counts[i] - 1)``."""
+    counts = counts.clamp(min=0)
+    total = int(counts.sum())
+    if total == 0:
+        return torch.empty(0, dtype=torch.int64, device=device), torch.empty(0, dtype=torch.int64, device=device)
+    idx = torch.repeat_interleave(torch.arange(len(counts), device=device), counts)
+    first = torch.cumsum(counts, 0) - counts
+    return idx, torch.arange(total, device=device) - first[idx] + starts[idx]
+
+
 def _toeplitz_pairs(L, l_toeplitz, l_exact, dl_band, device):
     """``(lo, hi)`` int32: the upper-triangle pairs whose exact values the Toeplitz fill reads
     (diagonal, column l_toeplitz, columns <= l_exact, dl_band diagonals;
-    ``gmaster/workspaces.py:520-539``), each once."""
-    l = np.arange(L, dtype=np.int64)
-    parts = [np.stack([l, np.full(L, l_toeplitz)])]
-    for c in range(l_exact + 1):
-        parts.append(np.stack([l, np.full(L, c)]))
-    for off in range(dl_band + 1):
-        parts.append(np.stack([l[: L - off], l[: L - off] + off]))
-    rc = np.concatenate(parts, axis=1)
-    lo, hi = np.minimum(rc[0], rc[1]), np.maximum(rc[0], rc[1])
-    enc = np.unique(lo * L + hi)
-    return (torch.as_tensor((enc // L).astype(np.int32), device=device),
-            torch.as_tensor((enc % L).astype(np.int32), device=device))
+    ``gmaster/workspaces.py:520-539``), each once, built on the device as disjoint runs:
+    rows lo <= l_exact (every hi >= lo), band rows lo > l_exact (hi <= lo + dl_band), and
+    what column l_toeplitz adds outside both."""
+    lt, le, dl = int(l_toeplitz), int(l_exact), int(dl_band)
+    ar = lambda a, b: torch.arange(a, max(a, b), dtype=torch.int64, device=device)
+    lo_a = ar(0, min(le, L - 1) + 1)
+    ia, hi_a = _runs(lo_a, L - lo_a, device)
+    lo_b = ar(le + 1, L)
+    ib, hi_b = _runs(lo_b, torch.clamp(L - lo_b, max=dl + 1), device)
+    lo_c1 = ar(le + 1, lt - dl)
+    hi_c2 = ar(lt + dl + 1, L) if lt > le else ar(0, 0)
+    lo = torch.cat([lo_a[ia], lo_b[ib], lo_c1, torch.full_like(hi_c2, lt)])
+    hi = torch.cat([hi_a, hi_b, torch.full_like(lo_c1, lt), hi_c2])
+    return lo.to(torch.int32), hi.to(torch.int32)
 
 
 def coupling_blocks(window, lmax, toeplitz=None):

@@ -181,6 +181,25 @@ int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, const int32
 int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
                 const int32_t* band_start, int64_t nbands, const double* w, double* out, void* stream);
 
+/* ---- mixed precision: FP32 contraction ---------------------------------------------------
+ * Same arguments as the FP64 entries above.  The recurrence, its tables and seeds stay
+ * FP64; each step's value is rounded to FP32 once and the contraction over rings and the
+ * block reductions run in FP32 (spectra read and partial sums / spectra written in FP64),
+ * ~1e-7 relative.  No counterpart in the reference, whose march is FP32 throughout
+ * (gmaster/_native/cuda/march_v2.cu:155-407). */
+int gm_lat_ana_s0_mixed(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside, int64_t lmax,
+                        const double* alpha, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+                        double* partial, int64_t nchunks, void* stream);
+int gm_lat_syn_s0_mixed(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
+                        const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, double* spec,
+                        int64_t ld, int64_t smap, void* stream);
+int gm_lat_ana_s_mixed(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
+                       const double* gamma, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+                       double* partial, int64_t nchunks, void* stream);
+int gm_lat_syn_s_mixed(const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
+                       const double* gamma, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+                       double* spec, int64_t ld, void* stream);
+
 /* ---- learned coupling surrogate: dense network kernels --------------------------------
  * No counterpart in the reference, whose coupling is exact or Toeplitz only
  * (gmaster/workspaces.py:542-590, :1141-1194); these train and run the network that predicts

@@ -1,0 +1,120 @@
+"""GPU checks of the mixed-precision form: FP32 refinement passes against the FP64 path and
+the oracle fixtures, and the fast coupling form."""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("nside,spin", [(16, 0), (64, 0), (64, 2), (128, 2)])
+def test_mixed_kernels_match_fp64_kernels_to_fp32_precision(nside, spin):
+    """One analysis and one synthesis pass with FP32 contraction against the FP64 kernels:
+    the recurrence is FP64 in both, so only the FP32 products and sums differ."""
+    from gmaster_amd import sht
+
+    print(f"mixed kernels nside={nside} spin={spin}", flush=True)
+    rng = np.random.default_rng(nside + spin)
+    lmax = 3 * nside - 1
+    L = lmax + 1
+    nelem = L * (L + 1) // 2
+    p = sht.plan(nside, lmax, spin, "cuda")
+    nring = 4 * nside - 1
+    if spin == 0:
+        spec = torch.as_tensor(rng.standard_normal((2, nring, L)) + 1j * rng.standard_normal((2, nring, L)),
+                               device="cuda")
+        a64 = torch.empty((2, nelem), dtype=torch.complex128, device="cuda")
+        a32 = torch.empty_like(a64)
+        sht._analysis_s0(p, spec, a64, 0)
+        sht._analysis_s0(p, spec, a32, 0, mixed=True)
+        s64, s32 = torch.empty_like(spec), torch.empty_like(spec)
+        sht._synthesis_s0(p, a64, s64)
+        sht._synthesis_s0(p, a64, s32, mixed=True)
+    else:
+        W = 2 * L - 1
+        spec = torch.as_tensor(rng.standard_normal((nring, W)) + 1j * rng.standard_normal((nring, W)), device="cuda")
+        a64 = torch.empty((2, nelem), dtype=torch.complex128, device="cuda")
+        a32 = torch.empty_like(a64)
+        sht._analysis_s(p, spec, a64[0], a64[1], 0)
+        sht._analysis_s(p, spec, a32[0], a32[1], 0, mixed=True)
+        s64, s32 = torch.empty_like(spec), torch.empty_like(spec)
+        sht._synthesis_s(p, a64[0], a64[1], s64)
+        sht._synthesis_s(p, a64[0], a64[1], s32, mixed=True)
+    ea, es = _rel(a32.cpu(), a64.cpu()), _rel(s32.cpu(), s64.cpu())
+    print(f"  analysis rel {ea:.2e} synthesis rel {es:.2e}", flush=True)
+    assert ea < 1e-5 and es < 1e-5
+
+
+@pytest.mark.parametrize("nside,spin", [(32, 0), (64, 0), (64, 2), (256, 2)])
+def test_mixed_refinement_matches_fp64_refinement(nside, spin):
+    from gmaster_amd import sht, workload
+
+    print(f"mixed refinement nside={nside} spin={spin}", flush=True)
+    lmax = 3 * nside - 1
+    maps = workload.tqu_maps(nside, lmax, seed=nside)
+    mask = workload.hole_mask(nside, device="cuda")
+    m = maps[0:1] * mask if spin == 0 else maps[1:3] * mask
+    a64 = sht.map2alm(m, spin, lmax, n_iter=3)
+    a32 = sht.map2alm(m, spin, lmax, n_iter=3, precision="mixed")
+    a0 = sht.map2alm(m, spin, lmax, n_iter=0)
+    e, step = _rel(a32.cpu(), a64.cpu()), _rel(a0.cpu(), a64.cpu())
+    print(f"  mixed vs fp64 rel {e:.2e} (refinement moves alm by {step:.2e})", flush=True)
+    assert e < 1e-6 and e < 1e-4 * step
+
+
+def test_fast_form_pipeline_matches_fixture():
+    import gmaster_amd as nmt
+    from gmaster_amd import field, workspaces
+
+    fx = np.load(os.path.join(GOLDEN, "pipeline_nside64.npz"))
+    workspaces.clear_coupling_cache()
+    field.clear_mask_cache()
+    nmt.set_form("fast")
+    try:
+        mask = torch.as_tensor(fx["mask"], device="cuda")
+        tqu = torch.as_tensor(fx["tqu"], device="cuda")
+        fT = nmt.NmtField(mask, tqu[0:1], n_iter=3)
+        fP = nmt.NmtField(mask, tqu[1:3], n_iter=3)
+        b = nmt.NmtBin.from_nside_linear(64, 16)
+        for name, (f1, f2) in {"00": (fT, fT), "02": (fT, fP), "22": (fP, fP)}.items():
+            w = nmt.NmtWorkspace.from_fields(f1, f2, b)
+            bp = np.asarray(w.decouple_cell(nmt.compute_coupled_cell(f1, f2)))
+            e = _rel(bp, fx["bp_" + name])
+            print(f"fast form bp_{name} rel {e:.2e} coupling {w.coupling_info}", flush=True)
+            assert e < 1e-6
+    finally:
+        nmt.set_form("fp64")
+        field.clear_mask_cache()
+        workspaces.clear_coupling_cache()
+
+
+def test_fast_coupling_judge_falls_back_when_tolerance_is_tiny():
+    from gmaster_amd import workload
+    from gmaster_amd.learned import mask_window
+    from gmaster_amd.workspaces import clear_coupling_cache, coupling_blocks, fast_coupling_blocks
+
+    nside = 256
+    lmax = 3 * nside - 1
+    win = mask_window(workload.hole_mask(nside, device="cuda"), nside, lmax)
+    clear_coupling_cache()
+    exact = {k: v.clone() for k, v in coupling_blocks(win, lmax).items()}
+    clear_coupling_cache()
+    approx, info = fast_coupling_blocks(win, lmax, toeplitz=(200, 32, 32), tol=1.0)
+    print(f"judge info {info}", flush=True)
+    assert not info["fallback"] and info["sample_err"] > 0
+    assert not torch.equal(approx["00"], exact["00"])
+    clear_coupling_cache()
+    back, info = fast_coupling_blocks(win, lmax, toeplitz=(200, 32, 32), tol=1e-14)
+    assert info["fallback"]
+    for k in exact:
+        assert torch.equal(back[k], exact[k])
