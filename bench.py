@@ -12,14 +12,15 @@ Workload (synthetic, seeded): mask = 1 except 20 random 5-degree disc holes, C1-
 2 degrees; T, Q, U Gaussian with C_TT = 400/(l+10)^2, C_EE = 0.1 C_TT, C_TE = 0.3
 sqrt(TT EE), C_BB = 0.01 C_EE; lmax = 3 nside - 1; bins of 16.
 
-Two forms are timed in the same run: plain FP64 (exact 3j coupling) and the chosen form
-(``--form``, default ``fast``: the coupling from the Toeplitz approximation with a wide exact
-region, judged by exact spot checks; every transform stays FP64).  ``value`` is the chosen
-form's time; ``fp64_s`` the plain form's, and ``bandpower_err_vs_fp64`` the chosen form's
-largest relative bandpower error per spectrum.
+Two forms are timed in the same run: plain FP64 and the chosen form (``--form``, default
+``fast``): Legendre passes with an FP64 recurrence and FP32 contraction, the Jacobi
+refinement on corrections, and the coupling from the Toeplitz approximation with a wide
+exact region judged by exact spot checks.  ``value`` is the chosen form's time; ``fp64_s``
+the plain form's, and ``bandpower_err_vs_fp64`` the chosen form's largest relative
+bandpower error per spectrum.
 
 Prints one JSON line (value = seconds per pipeline, lower is better) with a roofline
-entry for the dominant kernel (spin-2 Legendre analysis, FP64 VALU bound) measured with
+entry for the chosen form's dominant kernel (spin-2 Legendre analysis) measured with
 HIP events on the launching stream during the timed steps, and a CPU baseline (the
 project's C/OpenMP FP64 restatement of the same Legendre and 3j work, timed on a bounded
 sample of orders / rows at the same Nside and extrapolated to the whole pipeline).
@@ -36,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 vector / matrix, FLOP/s
+PEAK_FP32 = 157.3e12  # MI355X FP32 vector (unpacked) / matrix, FLOP/s
 
 
 def log(msg):
@@ -55,10 +57,11 @@ def legendre_flops(nside, lmax):
     return 2 * (2 + 4) * T0, 2 * (2 + 4) * T2
 
 
-def _measured_traffic(nside):
+def _measured_traffic(nside, mixed=False):
     """HBM bytes per k_ana_s launch from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE
     summary at this Nside (copy-kernel scaled), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"k_ana_s_traffic_nside{nside}.txt")
+    name = "k_ana_s_mixed" if mixed else "k_ana_s"
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic_nside{nside}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -177,38 +180,51 @@ def main():
         return times, bp, timers, list(info)
 
     forms = ["fp64"] if args.form == "fp64" else ["fp64", args.form]
-    runs = {f: timed(f, f == "fp64") for f in forms}
-    times_fp64, bp_fp64, timers, _ = runs["fp64"]
-    times, bp, _, coupling_info = runs[args.form]
+    runs = {f: timed(f, True) for f in forms}
+    times_fp64, bp_fp64, timers_fp64, _ = runs["fp64"]
+    times, bp, timers, coupling_info = runs[args.form]
     total = sum(times)
     bp_err = bandpower_errors(bp, bp_fp64) if args.form != "fp64" else None
     if bp_err is not None:
         log(f"{args.form} vs fp64 bandpower errors: {bp_err}")
 
-    # per entry point: mean duration per launch (ms) over the timed steps
-    stages = {}
-    for name, evs in timers.items():
-        ms = [s.elapsed_time(e) for s, e in evs]
-        stages[name] = {"calls": len(ms), "mean_ms": sum(ms) / len(ms), "total_ms_per_step": sum(ms) / args.steps}
+    def per_call(tm):
+        # per entry point: mean duration per launch (ms) over the timed steps
+        out = {}
+        for name, evs in tm.items():
+            ms = [s_.elapsed_time(e) for s_, e in evs]
+            out[name] = {"calls": len(ms), "mean_ms": sum(ms) / len(ms), "total_ms_per_step": sum(ms) / args.steps}
+        return out
+
+    stages, stages_fp64 = per_call(timers), per_call(timers_fp64)
     fp64_s = sum(times_fp64) / len(times_fp64)
     f0, f2 = legendre_flops(nside, lmax)
-    dom = "gm_lat_ana_s"
-    dom_ms = stages.get(dom, {}).get("mean_ms", float("nan"))
-    achieved = f2 / (dom_ms * 1e-3)
-    roofline = {
-        "kernel": "k_ana_s (spin-2 Legendre analysis, gm_lat_ana_s)",
-        "bound": "mfma",
-        "pipe": "FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)",
-        "achieved": achieved / 1e12,
-        "peak": PEAK_FP64 / 1e12,
-        "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP64,
-        "traffic": _measured_traffic(nside),
-        "flop_per_launch": f2,
-        "mean_launch_ms": dom_ms,
-    }
-    ana0_ms = stages.get("gm_lat_ana_s0", {}).get("mean_ms", float("nan"))
-    cpl_ms = stages.get("gm_coupling", {}).get("mean_ms", float("nan"))
+
+    def roof(st, mixed):
+        name = "gm_lat_ana_s_mixed" if mixed else "gm_lat_ana_s"
+        dom_ms = st.get(name, {}).get("mean_ms", float("nan"))
+        achieved = f2 / (dom_ms * 1e-3)
+        # 2 (2 + 4) FLOP per (l, m, ring): 4 of the recurrence (FP64), 8 of the contraction
+        # (FP64, or FP32 in the mixed kernel); the peak is the time-weighted one of that mix
+        peak = PEAK_FP64 if not mixed else 12.0 / (4.0 / PEAK_FP64 + 8.0 / PEAK_FP32)
+        return {
+            "kernel": f"k_ana_s{'<mixed>' if mixed else ''} (spin-2 Legendre analysis, {name})",
+            "bound": "mfma",
+            "pipe": ("FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)" if not mixed else
+                     "FP64 VALU recurrence (78.6 TFLOP/s) + FP32 VALU contraction (157.3 TFLOP/s): op-mix peak"),
+            "achieved": achieved / 1e12,
+            "peak": peak / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": _measured_traffic(nside, mixed),
+            "flop_per_launch": f2,
+            "mean_launch_ms": dom_ms,
+        }
+
+    roofline = roof(stages, args.form != "fp64")
+    roofline_fp64 = roof(stages_fp64, False)
+    ana0_ms = stages_fp64.get("gm_lat_ana_s0", {}).get("mean_ms", float("nan"))
+    cpl_ms = stages_fp64.get("gm_coupling", {}).get("mean_ms", float("nan"))
 
     cpu = None
     if not args.no_cpu:
@@ -242,10 +258,12 @@ def main():
                                f"20 x 5deg holes C1 2deg, bins of {nlb}", "nside": nside, "lmax": lmax,
                    "n_iter": 3, "nlb": nlb, "parallelism": "single GPU"},
         "roofline": roofline,
+        "roofline_fp64": roofline_fp64,
         "stages": {
             "ana_s0_pair_fp64_frac": (f0 / (ana0_ms * 1e-3)) / PEAK_FP64,
-            "coupling_ms": cpl_ms,
+            "coupling_fp64_ms": cpl_ms,
             "per_call": stages,
+            "per_call_fp64": stages_fp64,
         },
         "bandpowers_finite": bool(all(torch.isfinite(x).all() for x in bp)),
         "timed_steps_s": times,

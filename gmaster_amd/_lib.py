@@ -29,12 +29,14 @@ SIGNATURES = {
     "gm_lat_ana_s0": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_ana_s0_mixed": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_finish_s0": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
+    "gm_lat_finish_s0_mixed": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
     "gm_lat_prep_s0": [_p, _i64, _i64, _p, _i64, _p, _p],
     "gm_lat_syn_s0": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
     "gm_lat_syn_s0_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
     "gm_lat_ana_s": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_ana_s_mixed": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_finish_s": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _p],
+    "gm_lat_finish_s_mixed": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _p],
     "gm_lat_prep_s": [_p, _p, _p, _i64, _i64, _p, _p],
     "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_syn_s_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
@@ -53,11 +55,12 @@ SIGNATURES = {
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
     "gm_table_size": [_i64],
     "gm_lat_nchunks": [_i64, _i64],
+    "gm_lat_nchunks_mixed": [_i64, _i64],
     "gm_cl_work_size": [_i64, _i64],
     "gm_abi_version": [],
     "gm_last_error": [],
 }
-_RET_I64 = {"gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
+_RET_I64 = {"gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
 
 _lib = None
 

@@ -17,6 +17,8 @@
 // Rings are ordered from the equator to the poles and the orders m whose polar-skip
 // bound mlim(theta) (ducc0's rule, docs/notes/latitudinal_march_maths.md:245-250) is
 // exceeded are simply not visited: nactive[m] is a prefix length.
+#include <type_traits>
+
 #include "common.h"
 
 namespace gm {
@@ -653,18 +655,67 @@ __global__ void k_prep_s(const double2* __restrict__ ae, const double2* __restri
 // Large geometries use 4-wave workgroups with R rings per lane: R = 6 for spin 0 (two
 // maps' E/O coefficients per ring keep a wave under 256 VGPRs, i.e. two waves per SIMD),
 // R = 8 for spin s.
-LatConfig lat_config(int64_t nside, int64_t spin) {
+// Rings per lane of the large geometries, per precision and pass (FP32-contraction kernels
+// hold half the registers per ring).  Chosen by timing R = 3..8 on MI355X at Nside 4096:
+// occupancy steps make the best R differ between passes.
+#ifndef GM_ANA_R_S0
+#define GM_ANA_R_S0 6
+#endif
+#ifndef GM_SYN_R_S0
+#define GM_SYN_R_S0 4
+#endif
+#ifndef GM_ANA_R_S
+#define GM_ANA_R_S 8
+#endif
+#ifndef GM_SYN_R_S
+#define GM_SYN_R_S 6
+#endif
+#ifndef GM_MIX_ANA_R_S0
+#define GM_MIX_ANA_R_S0 5
+#endif
+#ifndef GM_MIX_SYN_R_S0
+#define GM_MIX_SYN_R_S0 5
+#endif
+#ifndef GM_MIX_ANA_R_S
+#define GM_MIX_ANA_R_S 6
+#endif
+#ifndef GM_MIX_SYN_R_S
+#define GM_MIX_SYN_R_S 5
+#endif
+
+constexpr int large_r(bool syn, bool mixed, bool spin0) {
+    return syn ? (mixed ? (spin0 ? GM_MIX_SYN_R_S0 : GM_MIX_SYN_R_S) : (spin0 ? GM_SYN_R_S0 : GM_SYN_R_S))
+               : (mixed ? (spin0 ? GM_MIX_ANA_R_S0 : GM_MIX_ANA_R_S) : (spin0 ? GM_ANA_R_S0 : GM_ANA_R_S));
+}
+
+LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn) {
     const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
     LatConfig cfg;
+    // the launchers select the instantiation by NW: NW 1 -> R 2, NW 2 -> R 4, NW 4 -> the
+    // large R; CH = 64 NW R must match the kernel's own chunk size
     if (nrings <= 128) cfg = {2, 1, 128, 0};
     else if (nrings <= 512) cfg = {4, 2, 512, 0};
-    else if (spin == 0) cfg = {6, 4, 1536, 0};
-    else cfg = {8, 4, 2048, 0};
+    else {
+        const int r = large_r(syn, mixed, spin == 0);
+        cfg = {r, 4, 256 * r, 0};
+    }
     cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
     return cfg;
 }
 
-LatConfig syn_config(int64_t nside, int64_t spin) { return lat_config(nside, spin); }
+LatConfig lat_config(int64_t nside, int64_t spin, bool mixed = false) { return make_config(nside, spin, mixed, false); }
+
+LatConfig syn_config(int64_t nside, int64_t spin, bool mixed = false) { return make_config(nside, spin, mixed, true); }
+
+// The launchers pair NW 1 with R 2, NW 2 with R 4 and NW 4 with the large R; a config that
+// does not match its instantiation would run a kernel on the wrong number of threads.
+int check_config(const char* name, const LatConfig& cfg, int large_r) {
+    const int r = cfg.NW == 1 ? 2 : cfg.NW == 2 ? 4 : cfg.NW == 4 ? large_r : -1;
+    if (r != cfg.R || cfg.CH != 64 * cfg.NW * cfg.R)
+        return fail(GM_E_LAUNCH, "%s: internal launch configuration mismatch (R %d, NW %d, CH %d)", name, cfg.R, cfg.NW,
+                    cfg.CH);
+    return GM_OK;
+}
 
 int check_lat(int64_t nside, int64_t lmax, int64_t spin) {
     if (nside < 1 || (nside & (nside - 1))) return fail(GM_E_ARG, "nside must be a power of 2");
@@ -682,6 +733,8 @@ using namespace gm;
 extern "C" int64_t gm_table_size(int64_t lmax) { return table_size(lmax + 1); }
 
 extern "C" int64_t gm_lat_nchunks(int64_t nside, int64_t spin) { return lat_config(nside, spin).nchunks; }
+
+extern "C" int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin) { return lat_config(nside, spin, true).nchunks; }
 
 extern "C" int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* gamma, double* P, void* stream) {
     if (lmax < 0 || spin < 0 || spin > lmax) return fail(GM_E_ARG, "gm_lat_tables: bad lmax / spin");
@@ -706,22 +759,24 @@ int launch_ana_s0(const char* name, const double* spec, int64_t nmaps, int64_t l
     const int64_t L = lmax + 1;
     if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
     if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
-    const LatConfig cfg = lat_config(nside, 0);
+    const LatConfig cfg = lat_config(nside, 0, std::is_same<F, float>::value);
     if (nchunks != cfg.nchunks)
         return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
     const int64_t ts = table_size(L);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* sp = reinterpret_cast<const double2*>(spec);
+    constexpr int RL = large_r(false, std::is_same<F, float>::value, true);
+    if (int e = check_config(name, cfg, RL)) return e;
 #define GM_ANA0(KK, RR, WW) hipLaunchKernelGGL((k_ana_s0<KK, RR, WW, F>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts)
     if (nmaps == 1) {
-        if (cfg.R == 2) GM_ANA0(1, 2, 1);
-        else if (cfg.R == 4) GM_ANA0(1, 4, 2);
-        else GM_ANA0(1, 6, 4);
+        if (cfg.NW == 1) GM_ANA0(1, 2, 1);
+        else if (cfg.NW == 2) GM_ANA0(1, 4, 2);
+        else GM_ANA0(1, RL, 4);
     } else {
-        if (cfg.R == 2) GM_ANA0(2, 2, 1);
-        else if (cfg.R == 4) GM_ANA0(2, 4, 2);
-        else GM_ANA0(2, 6, 4);
+        if (cfg.NW == 1) GM_ANA0(2, 2, 1);
+        else if (cfg.NW == 2) GM_ANA0(2, 4, 2);
+        else GM_ANA0(2, RL, 4);
     }
 #undef GM_ANA0
     return check_launch(name);
@@ -737,20 +792,22 @@ int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t n
     const int64_t L = lmax + 1;
     if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
     if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
-    const LatConfig cfg = syn_config(nside, 0);
+    const LatConfig cfg = syn_config(nside, 0, std::is_same<F, float>::value);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* cf = reinterpret_cast<const double2*>(coef);
     double2* sp = reinterpret_cast<double2*>(spec);
+    constexpr int RL = large_r(true, std::is_same<F, float>::value, true);
+    if (int e = check_config(name, cfg, RL)) return e;
 #define GM_SYN0(KK, RR, WW) hipLaunchKernelGGL((k_syn_s0<KK, RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap)
     if (nmaps == 1) {
-        if (cfg.R == 2) GM_SYN0(1, 2, 1);
-        else if (cfg.R == 4) GM_SYN0(1, 4, 2);
-        else GM_SYN0(1, 6, 4);
+        if (cfg.NW == 1) GM_SYN0(1, 2, 1);
+        else if (cfg.NW == 2) GM_SYN0(1, 4, 2);
+        else GM_SYN0(1, RL, 4);
     } else {
-        if (cfg.R == 2) GM_SYN0(2, 2, 1);
-        else if (cfg.R == 4) GM_SYN0(2, 4, 2);
-        else GM_SYN0(2, 6, 4);
+        if (cfg.NW == 1) GM_SYN0(2, 2, 1);
+        else if (cfg.NW == 2) GM_SYN0(2, 4, 2);
+        else GM_SYN0(2, RL, 4);
     }
 #undef GM_SYN0
     return check_launch(name);
@@ -766,17 +823,19 @@ int launch_ana_s(const char* name, const double* spec, int64_t ld, int64_t nside
         return fail(GM_E_ARG, "%s: null pointer", name);
     const int64_t L = lmax + 1;
     if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "%s: ld < 2 lmax + 1", name);
-    const LatConfig cfg = lat_config(nside, spin);
+    const LatConfig cfg = lat_config(nside, spin, std::is_same<F, float>::value);
     if (nchunks != cfg.nchunks)
         return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* sp = reinterpret_cast<const double2*>(spec);
     const int64_t ts = table_size(L);
+    constexpr int RL = large_r(false, std::is_same<F, float>::value, false);
+    if (int e = check_config(name, cfg, RL)) return e;
 #define GM_ANAS(RR, WW) hipLaunchKernelGGL((k_ana_s<RR, WW, F>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts)
-    if (cfg.R == 2) GM_ANAS(2, 1);
-    else if (cfg.R == 4) GM_ANAS(4, 2);
-    else GM_ANAS(8, 4);
+    if (cfg.NW == 1) GM_ANAS(2, 1);
+    else if (cfg.NW == 2) GM_ANAS(4, 2);
+    else GM_ANAS(RL, 4);
 #undef GM_ANAS
     return check_launch(name);
 }
@@ -791,15 +850,17 @@ int launch_syn_s(const char* name, const double* coef, int64_t nside, int64_t lm
         return fail(GM_E_ARG, "%s: null pointer", name);
     const int64_t L = lmax + 1;
     if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "%s: ld < 2 lmax + 1", name);
-    const LatConfig cfg = syn_config(nside, spin);
+    const LatConfig cfg = syn_config(nside, spin, std::is_same<F, float>::value);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* cf = reinterpret_cast<const double2*>(coef);
     double2* sp = reinterpret_cast<double2*>(spec);
+    constexpr int RL = large_r(true, std::is_same<F, float>::value, false);
+    if (int e = check_config(name, cfg, RL)) return e;
 #define GM_SYNS(RR, WW) hipLaunchKernelGGL((k_syn_s<RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld)
-    if (cfg.R == 2) GM_SYNS(2, 1);
-    else if (cfg.R == 4) GM_SYNS(4, 2);
-    else GM_SYNS(8, 4);
+    if (cfg.NW == 1) GM_SYNS(2, 1);
+    else if (cfg.NW == 2) GM_SYNS(4, 2);
+    else GM_SYNS(RL, 4);
 #undef GM_SYNS
     return check_launch(name);
 }
@@ -821,19 +882,37 @@ extern "C" int gm_lat_ana_s0_mixed(const double* spec, int64_t nmaps, int64_t ld
                                 nactive, partial, nchunks, stream);
 }
 
-extern "C" int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
-                                const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
-                                void* stream) {
+namespace gm {
+namespace {
+int finish_s0(const char* name, bool mixed, const double* partial, int64_t nmaps, int64_t nside,
+              const int32_t* nactive, const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
+              void* stream) {
     if (int e = check_lat(nside, lmax, 0)) return e;
-    if (!partial || !nactive || !P || !alm) return fail(GM_E_ARG, "gm_lat_finish_s0: null pointer");
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_finish_s0: nmaps must be 1 or 2");
+    if (!partial || !nactive || !P || !alm) return fail(GM_E_ARG, "%s: null pointer", name);
+    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "%s: nmaps must be 1 or 2", name);
     const int64_t L = lmax + 1, nelem = L * (L + 1) / 2;
-    if (nmaps > 1 && alm_map_stride < nelem) return fail(GM_E_SHAPE, "gm_lat_finish_s0: alm stride too small");
-    const LatConfig cfg = lat_config(nside, 0);
+    if (nmaps > 1 && alm_map_stride < nelem) return fail(GM_E_SHAPE, "%s: alm stride too small", name);
+    const LatConfig cfg = lat_config(nside, 0, mixed);
     hipLaunchKernelGGL(k_finish_s0, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
                        reinterpret_cast<const double2*>(partial), (int)nmaps, cfg.nchunks, cfg.CH, nactive, P, (int)L,
                        table_size(L), reinterpret_cast<double2*>(alm), alm_map_stride, mode);
-    return check_launch("gm_lat_finish_s0");
+    return check_launch(name);
+}
+}  // namespace
+}  // namespace gm
+
+extern "C" int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
+                                const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
+                                void* stream) {
+    return finish_s0("gm_lat_finish_s0", false, partial, nmaps, nside, nactive, P, lmax, alm, alm_map_stride, mode,
+                     stream);
+}
+
+extern "C" int gm_lat_finish_s0_mixed(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
+                                      const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
+                                      void* stream) {
+    return finish_s0("gm_lat_finish_s0_mixed", true, partial, nmaps, nside, nactive, P, lmax, alm, alm_map_stride,
+                     mode, stream);
 }
 
 extern "C" int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_stride, const double* P,
@@ -876,16 +955,31 @@ extern "C" int gm_lat_ana_s_mixed(const double* spec, int64_t ld, int64_t nside,
                                nactive, partial, nchunks, stream);
 }
 
-extern "C" int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
-                               int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream) {
+namespace gm {
+namespace {
+int finish_s(const char* name, bool mixed, const double* partial, int64_t nside, const int32_t* nactive,
+             const double* P, int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream) {
     if (int e = check_lat(nside, lmax, spin)) return e;
-    if (!partial || !nactive || !P || !alm_e || !alm_b) return fail(GM_E_ARG, "gm_lat_finish_s: null pointer");
+    if (!partial || !nactive || !P || !alm_e || !alm_b) return fail(GM_E_ARG, "%s: null pointer", name);
     const int64_t L = lmax + 1;
-    const LatConfig cfg = lat_config(nside, spin);
+    const LatConfig cfg = lat_config(nside, spin, mixed);
     hipLaunchKernelGGL(k_finish_s, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
                        reinterpret_cast<const double2*>(partial), cfg.nchunks, cfg.CH, nactive, P, (int)L, (int)spin,
                        table_size(L), reinterpret_cast<double2*>(alm_e), reinterpret_cast<double2*>(alm_b), mode);
-    return check_launch("gm_lat_finish_s");
+    return check_launch(name);
+}
+}  // namespace
+}  // namespace gm
+
+extern "C" int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
+                               int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream) {
+    return finish_s("gm_lat_finish_s", false, partial, nside, nactive, P, lmax, spin, alm_e, alm_b, mode, stream);
+}
+
+extern "C" int gm_lat_finish_s_mixed(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
+                                     int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode,
+                                     void* stream) {
+    return finish_s("gm_lat_finish_s_mixed", true, partial, nside, nactive, P, lmax, spin, alm_e, alm_b, mode, stream);
 }
 
 extern "C" int gm_lat_prep_s(const double* alm_e, const double* alm_b, const double* P, int64_t lmax, int64_t spin,

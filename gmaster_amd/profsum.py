@@ -66,6 +66,11 @@ def pmc(path, counter):
         print(f"{k:40s} {len(v):10d} {sum(v) / len(v):16.1f} {max(v):16.1f}")
 
 
+def _is(name, kernel):
+    """``kernel`` is a bare name (any template arguments) or ``name<args>`` (exact)."""
+    return name == kernel if "<" in kernel else name.split("<")[0] == kernel
+
+
 def _copy_max(acc):
     vals = [max(v) for k, v in acc.items() if "copy" in k.lower()]
     if not vals:
@@ -80,7 +85,7 @@ def traffic(fetch_path, write_path, kernel, copy_bytes):
         acc = _per_dispatch(path, counter)
         ref_kib = _copy_max(acc)
         scale = copy_bytes / (ref_kib * 1024.0)
-        vals = [x for k, v in acc.items() if short(k).split("<")[0] == kernel for x in v]
+        vals = [x for k, v in acc.items() if _is(short(k), kernel) for x in v]
         if not vals:
             raise SystemExit(f"kernel {kernel} not found in {path}")
         mean_kib = sum(vals) / len(vals)
@@ -108,7 +113,7 @@ def traffic_from_summaries(fetch_txt, write_txt, kernel, copy_bytes):
     for tag, path, counter in (("fetch", fetch_txt, "FETCH_SIZE"), ("write", write_txt, "WRITE_SIZE")):
         rows = _summary_rows(path)
         ref_kib = max(r[3] for r in rows if "copy" in r[0].lower() and "rocclr" in r[0].lower())
-        hits = [r for r in rows if short(r[0]).split("<")[0] == kernel]
+        hits = [r for r in rows if _is(short(r[0]), kernel)]
         if not hits:
             raise SystemExit(f"kernel {kernel} not found in {path}")
         name, n, mean_kib, _ = hits[0]

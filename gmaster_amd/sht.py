@@ -99,6 +99,7 @@ class Plan:
     nact: torch.Tensor
     nchunks: int
     tsize: int
+    nchunks_mixed: int = 0
 
 
 def plan(nside, lmax, spin, device=None, polar_skip=True):
@@ -126,6 +127,7 @@ def plan(nside, lmax, spin, device=None, polar_skip=True):
         nact=torch.as_tensor(_nactive(nside, lmax, spin, polar_skip), dtype=torch.int32, device=device),
         nchunks=int(_lib.query("gm_lat_nchunks", nside, spin)),
         tsize=tsize,
+        nchunks_mixed=int(_lib.query("gm_lat_nchunks_mixed", nside, spin)),
     )
     _PLANS[key] = p
     return p
@@ -151,12 +153,13 @@ def _as_f64(t, device):
 def _analysis_s0(p, spec, alm, mode, mixed=False):
     K = spec.shape[0]
     st = _lib.stream_ptr(p.device)
-    partial = torch.empty(p.nchunks * p.tsize * 2 * K, dtype=torch.float64, device=p.device)
+    nch = p.nchunks_mixed if mixed else p.nchunks
+    sfx = "_mixed" if mixed else ""
+    partial = torch.empty(nch * p.tsize * 2 * K, dtype=torch.float64, device=p.device)
     ld = spec.shape[2]
-    name, alpha = ("gm_lat_ana_s0_mixed" if mixed else "gm_lat_ana_s0"), p.alpha
-    _lib.call(name, _lib.ptr(spec), K, ld, spec.stride(0), p.nside, p.lmax, _lib.ptr(alpha),
-              _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), p.nchunks, st)
-    _lib.call("gm_lat_finish_s0", _lib.ptr(partial), K, p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax,
+    _lib.call("gm_lat_ana_s0" + sfx, _lib.ptr(spec), K, ld, spec.stride(0), p.nside, p.lmax, _lib.ptr(p.alpha),
+              _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), nch, st)
+    _lib.call("gm_lat_finish_s0" + sfx, _lib.ptr(partial), K, p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax,
               _lib.ptr(alm), alm.stride(0), int(mode), st)
 
 
@@ -266,12 +269,12 @@ def alm2map_s0(alm, nside, lmax, polar_skip=True):
 # ------------------------------------------------------------------------------ spin s
 def _analysis_s(p, spec, ae, ab, mode, mixed=False):
     st = _lib.stream_ptr(p.device)
-    partial = torch.empty(p.nchunks * p.tsize * 4, dtype=torch.float64, device=p.device)
-    name, (alpha, gamma) = ("gm_lat_ana_s_mixed" if mixed else "gm_lat_ana_s"), (p.alpha, p.gamma)
-    _lib.call(name, _lib.ptr(spec), spec.shape[1], p.nside, p.lmax, p.spin, _lib.ptr(alpha),
-              _lib.ptr(gamma), _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial),
-              p.nchunks, st)
-    _lib.call("gm_lat_finish_s", _lib.ptr(partial), p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax, p.spin,
+    nch = p.nchunks_mixed if mixed else p.nchunks
+    sfx = "_mixed" if mixed else ""
+    partial = torch.empty(nch * p.tsize * 4, dtype=torch.float64, device=p.device)
+    _lib.call("gm_lat_ana_s" + sfx, _lib.ptr(spec), spec.shape[1], p.nside, p.lmax, p.spin, _lib.ptr(p.alpha),
+              _lib.ptr(p.gamma), _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), nch, st)
+    _lib.call("gm_lat_finish_s" + sfx, _lib.ptr(partial), p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax, p.spin,
               _lib.ptr(ae), _lib.ptr(ab), int(mode), st)
 
 

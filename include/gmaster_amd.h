@@ -187,6 +187,12 @@ int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
  * block reductions run in FP32 (spectra read and partial sums / spectra written in FP64),
  * ~1e-7 relative.  No counterpart in the reference, whose march is FP32 throughout
  * (gmaster/_native/cuda/march_v2.cu:155-407). */
+/* Chunk counts and finishes of the mixed analyses (their ring chunks differ from FP64's). */
+int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin);
+int gm_lat_finish_s0_mixed(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
+                           const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode, void* stream);
+int gm_lat_finish_s_mixed(const double* partial, int64_t nside, const int32_t* nactive, const double* P, int64_t lmax,
+                          int64_t spin, double* alm_e, double* alm_b, int mode, void* stream);
 int gm_lat_ana_s0_mixed(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside, int64_t lmax,
                         const double* alpha, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
                         double* partial, int64_t nchunks, void* stream);
