@@ -257,7 +257,8 @@ class Surrogate:
     def save(self, path):
         from safetensors.torch import save_file
 
-        t = {f"w{i}": w.cpu().contiguous() for i, w in enumerate(self.net.w)}
+        # the network runs on bf16 weights, so those are what is stored
+        t = {f"w{i}": w.to(torch.bfloat16).cpu().contiguous() for i, w in enumerate(self.net.w)}
         t.update({f"b{i}": b.cpu().contiguous() for i, b in enumerate(self.net.b)})
         t.update({"x_mean": self.x_mean.cpu(), "x_std": self.x_std.cpu(), "y_scale": self.y_scale.cpu()})
         meta = {"sizes": json.dumps(self.net.sizes), "grid": json.dumps(self.grid.to_json()),
@@ -273,7 +274,7 @@ class Surrogate:
             t = {k: f.get_tensor(k) for k in f.keys()}
         sizes = json.loads(md["sizes"])
         nl = len(sizes) - 1
-        net = MLP.from_state({"sizes": sizes, "w": [t[f"w{i}"] for i in range(nl)],
+        net = MLP.from_state({"sizes": sizes, "w": [t[f"w{i}"].float() for i in range(nl)],
                               "b": [t[f"b{i}"] for i in range(nl)]}, device=device)
         dev = torch.device(device)
         return cls(ResidualGrid.from_json(json.loads(md["grid"])), net, t["x_mean"].to(dev), t["x_std"].to(dev),
