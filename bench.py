@@ -115,6 +115,52 @@ def bandpower_errors(bp, ref):
     return out
 
 
+def sht_batch(args):
+    """Per-map throughput of spin-0 map2alm (n_iter 3) + alm2map: ``--sht-batch K`` maps
+    sharing one recurrence (mixed precision) against the same maps one by one in FP64."""
+    import torch
+
+    from gmaster_amd import sht, workload
+
+    nside = args.nside
+    lmax = 3 * nside - 1
+    K = args.sht_batch
+    dev = torch.device("cuda", 0)
+    mask = workload.hole_mask(nside, device=dev)
+    maps = torch.stack([workload.tqu_maps(nside, lmax, seed=s)[0] * mask for s in range(K)])
+    torch.cuda.synchronize()
+
+    def run(batched):
+        if batched:
+            a = sht.map2alm_s0(maps, lmax, n_iter=3, precision="mixed", max_group=args.max_group)
+            return a, sht.alm2map_s0(a, nside, lmax, precision="mixed", max_group=args.max_group)
+        a = torch.cat([sht.map2alm_s0(maps[i:i + 1], lmax, n_iter=3) for i in range(K)])
+        return a, torch.cat([sht.alm2map_s0(a[i:i + 1], nside, lmax) for i in range(K)])
+
+    out = {}
+    for batched in (False, True):
+        for _ in range(args.warmup):
+            run(batched)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            res = run(batched)
+        torch.cuda.synchronize()
+        out[batched] = ((time.perf_counter() - t) / args.steps, res)
+        log(f"{'batched mixed' if batched else 'single FP64'}: {out[batched][0]:.3f}s for {K} maps")
+    a_s, a_b = out[False][1][0], out[True][1][0]
+    err = float(torch.max(torch.abs(a_b - a_s)) / torch.max(torch.abs(a_s)))
+    print(json.dumps({
+        "metric": f"spin-0 map2alm(n_iter 3) + alm2map maps/s, HEALPix Nside {nside}",
+        "value": K / out[True][0], "unit": "maps/s", "higher_is_better": True, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "dtype": "f64 recurrence, f32 contraction",
+        "data": "synthetic", "config": {"workload": f"{K} Gaussian maps x hole mask, Nside {nside}, lmax {lmax}",
+                                         "batch": K, "groups": [k for _, k in sht._groups(K, True, args.max_group)]},
+        "single_fp64_maps_per_s": K / out[False][0], "speedup": out[False][0] / out[True][0],
+        "alm_rel_err_vs_fp64": err,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,11 +174,17 @@ def main():
                     help="form timed for the metric (fp64 is always timed beside it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scales PMC byte counters)")
+    ap.add_argument("--sht-batch", type=int, default=0,
+                    help="instead of the MASTER pipeline, time K spin-0 maps through the batched transforms")
+    ap.add_argument("--max-group", type=int, default=2, help="maps per shared recurrence for --sht-batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
               f"(got --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+
+    if args.sht_batch > 0:
+        return sht_batch(args)
 
     import torch
 

@@ -55,7 +55,7 @@ SIGNATURES = {
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
     "gm_table_size": [_i64],
     "gm_lat_nchunks": [_i64, _i64],
-    "gm_lat_nchunks_mixed": [_i64, _i64],
+    "gm_lat_nchunks_mixed": [_i64, _i64, _i64],
     "gm_cl_work_size": [_i64, _i64],
     "gm_abi_version": [],
     "gm_last_error": [],

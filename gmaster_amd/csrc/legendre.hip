@@ -257,11 +257,16 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
         double an[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) an[i] = al[m + i + 1];
-        cpx<F> cn[4][K];
+        // one or two maps: the next block's coefficients are prefetched; for batches the
+        // 4K uniform values would not fit the scalar registers twice, so they load in place
+        constexpr int KP = K <= 2 ? K : 1;
+        cpx<F> cn[4][KP];
+        if constexpr (K <= 2) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int k = 0; k < K; ++k) cn[i][k] = load_c<F>(cf + (int64_t)(m + i) * K + k);
+                for (int k = 0; k < K; ++k) cn[i][k] = load_c<F>(cf + (int64_t)(m + i) * K + k);
+        }
         for (int l0 = m; l0 < L; l0 += 4) {
             double a[4];
             cpx<F> cb[4][K];
@@ -271,8 +276,12 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s0(const double2* __restrict__ 
                 an[i] = al[l0 + 4 + i + 1];
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    cb[i][k] = cn[i][k];
-                    cn[i][k] = load_c<F>(cf + (int64_t)(l0 + 4 + i) * K + k);  // next block (zero padded)
+                    if constexpr (K <= 2) {
+                        cb[i][k] = cn[i][k];
+                        cn[i][k] = load_c<F>(cf + (int64_t)(l0 + 4 + i) * K + k);  // next block (zero padded)
+                    } else {
+                        cb[i][k] = load_c<F>(cf + (int64_t)(l0 + i) * K + k);
+                    }
                 }
             }
 #pragma unroll
@@ -683,12 +692,16 @@ __global__ void k_prep_s(const double2* __restrict__ ae, const double2* __restri
 #define GM_MIX_SYN_R_S 5
 #endif
 
-constexpr int large_r(bool syn, bool mixed, bool spin0) {
-    return syn ? (mixed ? (spin0 ? GM_MIX_SYN_R_S0 : GM_MIX_SYN_R_S) : (spin0 ? GM_SYN_R_S0 : GM_SYN_R_S))
-               : (mixed ? (spin0 ? GM_MIX_ANA_R_S0 : GM_MIX_ANA_R_S) : (spin0 ? GM_ANA_R_S0 : GM_ANA_R_S));
+// Batched spin-0 maps (K = 4 or 8 sharing one recurrence, mixed precision only) hold 4K
+// FP32 coefficients per ring, so they take 3 (K = 4) or 2 (K = 8) rings per lane.
+constexpr int large_r(bool syn, bool mixed, bool spin0, int K = 1) {
+    return (spin0 && K >= 8) ? 2
+           : (spin0 && K >= 4) ? 3
+           : syn ? (mixed ? (spin0 ? GM_MIX_SYN_R_S0 : GM_MIX_SYN_R_S) : (spin0 ? GM_SYN_R_S0 : GM_SYN_R_S))
+                 : (mixed ? (spin0 ? GM_MIX_ANA_R_S0 : GM_MIX_ANA_R_S) : (spin0 ? GM_ANA_R_S0 : GM_ANA_R_S));
 }
 
-LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn) {
+LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn, int K = 1) {
     const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
     LatConfig cfg;
     // the launchers select the instantiation by NW: NW 1 -> R 2, NW 2 -> R 4, NW 4 -> the
@@ -696,16 +709,20 @@ LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn) {
     if (nrings <= 128) cfg = {2, 1, 128, 0};
     else if (nrings <= 512) cfg = {4, 2, 512, 0};
     else {
-        const int r = large_r(syn, mixed, spin == 0);
+        const int r = large_r(syn, mixed, spin == 0, K);
         cfg = {r, 4, 256 * r, 0};
     }
     cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
     return cfg;
 }
 
-LatConfig lat_config(int64_t nside, int64_t spin, bool mixed = false) { return make_config(nside, spin, mixed, false); }
+LatConfig lat_config(int64_t nside, int64_t spin, bool mixed = false, int K = 1) {
+    return make_config(nside, spin, mixed, false, K);
+}
 
-LatConfig syn_config(int64_t nside, int64_t spin, bool mixed = false) { return make_config(nside, spin, mixed, true); }
+LatConfig syn_config(int64_t nside, int64_t spin, bool mixed = false, int K = 1) {
+    return make_config(nside, spin, mixed, true, K);
+}
 
 // The launchers pair NW 1 with R 2, NW 2 with R 4 and NW 4 with the large R; a config that
 // does not match its instantiation would run a kernel on the wrong number of threads.
@@ -734,7 +751,9 @@ extern "C" int64_t gm_table_size(int64_t lmax) { return table_size(lmax + 1); }
 
 extern "C" int64_t gm_lat_nchunks(int64_t nside, int64_t spin) { return lat_config(nside, spin).nchunks; }
 
-extern "C" int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin) { return lat_config(nside, spin, true).nchunks; }
+extern "C" int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin, int64_t nmaps) {
+    return lat_config(nside, spin, true, (int)nmaps).nchunks;
+}
 
 extern "C" int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* gamma, double* P, void* stream) {
     if (lmax < 0 || spin < 0 || spin > lmax) return fail(GM_E_ARG, "gm_lat_tables: bad lmax / spin");
@@ -755,28 +774,40 @@ int launch_ana_s0(const char* name, const double* spec, int64_t nmaps, int64_t l
                   const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
     if (int e = check_lat(nside, lmax, 0)) return e;
     if (!spec || !alpha || !seed_mant || !seed_exp || !nactive || !partial) return fail(GM_E_ARG, "%s: null pointer", name);
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "%s: nmaps must be 1 or 2", name);
+    constexpr bool mixed = std::is_same<F, float>::value;
+    if (!(nmaps == 1 || nmaps == 2 || (mixed && (nmaps == 4 || nmaps == 8))))
+        return fail(GM_E_ARG, mixed ? "%s: nmaps must be 1, 2, 4 or 8" : "%s: nmaps must be 1 or 2", name);
     const int64_t L = lmax + 1;
     if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
     if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
-    const LatConfig cfg = lat_config(nside, 0, std::is_same<F, float>::value);
+    const LatConfig cfg = lat_config(nside, 0, mixed, (int)nmaps);
     if (nchunks != cfg.nchunks)
         return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
     const int64_t ts = table_size(L);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* sp = reinterpret_cast<const double2*>(spec);
-    constexpr int RL = large_r(false, std::is_same<F, float>::value, true);
-    if (int e = check_config(name, cfg, RL)) return e;
+    constexpr int RL = large_r(false, mixed, true);
+    if (int e = check_config(name, cfg, large_r(false, mixed, true, (int)nmaps))) return e;
 #define GM_ANA0(KK, RR, WW) hipLaunchKernelGGL((k_ana_s0<KK, RR, WW, F>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts)
     if (nmaps == 1) {
         if (cfg.NW == 1) GM_ANA0(1, 2, 1);
         else if (cfg.NW == 2) GM_ANA0(1, 4, 2);
         else GM_ANA0(1, RL, 4);
-    } else {
+    } else if (nmaps == 2) {
         if (cfg.NW == 1) GM_ANA0(2, 2, 1);
         else if (cfg.NW == 2) GM_ANA0(2, 4, 2);
         else GM_ANA0(2, RL, 4);
+    } else if constexpr (mixed) {
+        if (nmaps == 4) {
+            if (cfg.NW == 1) GM_ANA0(4, 2, 1);
+            else if (cfg.NW == 2) GM_ANA0(4, 4, 2);
+            else GM_ANA0(4, 3, 4);
+        } else {
+            if (cfg.NW == 1) GM_ANA0(8, 2, 1);
+            else if (cfg.NW == 2) GM_ANA0(8, 4, 2);
+            else GM_ANA0(8, 2, 4);
+        }
     }
 #undef GM_ANA0
     return check_launch(name);
@@ -788,26 +819,38 @@ int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t n
                   int64_t smap, void* stream) {
     if (int e = check_lat(nside, lmax, 0)) return e;
     if (!coef || !alpha || !seed_mant || !seed_exp || !nactive || !spec) return fail(GM_E_ARG, "%s: null pointer", name);
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "%s: nmaps must be 1 or 2", name);
+    constexpr bool mixed = std::is_same<F, float>::value;
+    if (!(nmaps == 1 || nmaps == 2 || (mixed && (nmaps == 4 || nmaps == 8))))
+        return fail(GM_E_ARG, mixed ? "%s: nmaps must be 1, 2, 4 or 8" : "%s: nmaps must be 1 or 2", name);
     const int64_t L = lmax + 1;
     if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
     if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
-    const LatConfig cfg = syn_config(nside, 0, std::is_same<F, float>::value);
+    const LatConfig cfg = syn_config(nside, 0, mixed, (int)nmaps);
     dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* cf = reinterpret_cast<const double2*>(coef);
     double2* sp = reinterpret_cast<double2*>(spec);
-    constexpr int RL = large_r(true, std::is_same<F, float>::value, true);
-    if (int e = check_config(name, cfg, RL)) return e;
+    constexpr int RL = large_r(true, mixed, true);
+    if (int e = check_config(name, cfg, large_r(true, mixed, true, (int)nmaps))) return e;
 #define GM_SYN0(KK, RR, WW) hipLaunchKernelGGL((k_syn_s0<KK, RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap)
     if (nmaps == 1) {
         if (cfg.NW == 1) GM_SYN0(1, 2, 1);
         else if (cfg.NW == 2) GM_SYN0(1, 4, 2);
         else GM_SYN0(1, RL, 4);
-    } else {
+    } else if (nmaps == 2) {
         if (cfg.NW == 1) GM_SYN0(2, 2, 1);
         else if (cfg.NW == 2) GM_SYN0(2, 4, 2);
         else GM_SYN0(2, RL, 4);
+    } else if constexpr (mixed) {
+        if (nmaps == 4) {
+            if (cfg.NW == 1) GM_SYN0(4, 2, 1);
+            else if (cfg.NW == 2) GM_SYN0(4, 4, 2);
+            else GM_SYN0(4, 3, 4);
+        } else {
+            if (cfg.NW == 1) GM_SYN0(8, 2, 1);
+            else if (cfg.NW == 2) GM_SYN0(8, 4, 2);
+            else GM_SYN0(8, 2, 4);
+        }
     }
 #undef GM_SYN0
     return check_launch(name);
@@ -889,10 +932,11 @@ int finish_s0(const char* name, bool mixed, const double* partial, int64_t nmaps
               void* stream) {
     if (int e = check_lat(nside, lmax, 0)) return e;
     if (!partial || !nactive || !P || !alm) return fail(GM_E_ARG, "%s: null pointer", name);
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "%s: nmaps must be 1 or 2", name);
+    if (!(nmaps == 1 || nmaps == 2 || (mixed && (nmaps == 4 || nmaps == 8))))
+        return fail(GM_E_ARG, mixed ? "%s: nmaps must be 1, 2, 4 or 8" : "%s: nmaps must be 1 or 2", name);
     const int64_t L = lmax + 1, nelem = L * (L + 1) / 2;
     if (nmaps > 1 && alm_map_stride < nelem) return fail(GM_E_SHAPE, "%s: alm stride too small", name);
-    const LatConfig cfg = lat_config(nside, 0, mixed);
+    const LatConfig cfg = lat_config(nside, 0, mixed, (int)nmaps);
     hipLaunchKernelGGL(k_finish_s0, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
                        reinterpret_cast<const double2*>(partial), (int)nmaps, cfg.nchunks, cfg.CH, nactive, P, (int)L,
                        table_size(L), reinterpret_cast<double2*>(alm), alm_map_stride, mode);
@@ -918,8 +962,10 @@ extern "C" int gm_lat_finish_s0_mixed(const double* partial, int64_t nmaps, int6
 extern "C" int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_stride, const double* P,
                               int64_t lmax, double* coef, void* stream) {
     if (lmax < 0 || !alm || !P || !coef) return fail(GM_E_ARG, "gm_lat_prep_s0: bad arguments");
-    if (nmaps != 1 && nmaps != 2) return fail(GM_E_ARG, "gm_lat_prep_s0: nmaps must be 1 or 2");
+    if (nmaps != 1 && nmaps != 2 && nmaps != 4 && nmaps != 8)
+        return fail(GM_E_ARG, "gm_lat_prep_s0: nmaps must be 1, 2, 4 or 8");
     const int64_t L = lmax + 1;
+    if (nmaps > 1 && alm_map_stride < L * (L + 1) / 2) return fail(GM_E_SHAPE, "gm_lat_prep_s0: alm stride too small");
     hipLaunchKernelGGL(k_prep_s0, dim3((unsigned)((L + GM_TABLE_PAD + 255) / 256), (unsigned)L), dim3(256), 0,
                        (hipStream_t)stream, reinterpret_cast<const double2*>(alm), (int)nmaps, alm_map_stride, P,
                        (int)L, reinterpret_cast<double2*>(coef));

@@ -127,7 +127,7 @@ def plan(nside, lmax, spin, device=None, polar_skip=True):
         nact=torch.as_tensor(_nactive(nside, lmax, spin, polar_skip), dtype=torch.int32, device=device),
         nchunks=int(_lib.query("gm_lat_nchunks", nside, spin)),
         tsize=tsize,
-        nchunks_mixed=int(_lib.query("gm_lat_nchunks_mixed", nside, spin)),
+        nchunks_mixed=int(_lib.query("gm_lat_nchunks_mixed", nside, spin, 1)),
     )
     _PLANS[key] = p
     return p
@@ -153,7 +153,7 @@ def _as_f64(t, device):
 def _analysis_s0(p, spec, alm, mode, mixed=False):
     K = spec.shape[0]
     st = _lib.stream_ptr(p.device)
-    nch = p.nchunks_mixed if mixed else p.nchunks
+    nch = int(_lib.query("gm_lat_nchunks_mixed", p.nside, 0, K)) if mixed else p.nchunks
     sfx = "_mixed" if mixed else ""
     partial = torch.empty(nch * p.tsize * 2 * K, dtype=torch.float64, device=p.device)
     ld = spec.shape[2]
@@ -211,14 +211,32 @@ def _refine(alm, n_iter, analysis, synthesis, fold, spec, mixed):
     return alm
 
 
-def map2alm_s0(maps, lmax, n_iter=0, polar_skip=True, precision="fp64"):
-    """``(K, npix)`` real maps (K = 1 or 2, sharing one recurrence) -> ``(K, nelem)`` alms.
-    ``precision="mixed"`` runs every pass with FP32 contraction (FP64 recurrence)."""
+def _groups(n, mixed, max_group=2):
+    """Split n maps into recurrence-sharing groups of at most ``max_group`` (2 by default;
+    4 or 8 in mixed precision: measured slower per map at Nside 1024, register and LDS
+    pressure lowering occupancy more than the shared recurrence saves)."""
+    sizes = tuple(k for k in ((8, 4, 2, 1) if mixed else (2, 1)) if k <= max_group)
+    out, start = [], 0
+    while start < n:
+        k = next(s for s in sizes if s <= n - start)
+        out.append((start, k))
+        start += k
+    return out
+
+
+def map2alm_s0(maps, lmax, n_iter=0, polar_skip=True, precision="fp64", max_group=2):
+    """``(K, npix)`` real maps -> ``(K, nelem)`` alms.  Maps share one recurrence in groups
+    of up to ``max_group`` (2 in FP64; 2, 4 or 8 with ``precision="mixed"``, which runs every
+    pass with FP32 contraction over the FP64 recurrence)."""
     maps = torch.as_tensor(maps)
     dev = maps.device if maps.is_cuda else torch.device("cuda")
     maps = _as_f64(maps, dev)
-    if maps.ndim != 2 or maps.shape[0] not in (1, 2):
+    if maps.ndim != 2 or maps.shape[0] < 1:
         raise ValueError("Input map has wrong shape")
+    groups = _groups(maps.shape[0], _mixed(precision), max_group)
+    if len(groups) > 1:
+        return torch.cat([map2alm_s0(maps[a:a + k], lmax, n_iter, polar_skip, precision, max_group)
+                          for a, k in groups])
     K, npix = maps.shape
     nside = _check_nside(npix)
     lmax = int(lmax)
@@ -246,19 +264,23 @@ def _mixed(precision):
     return precision == "mixed"
 
 
-def alm2map_s0(alm, nside, lmax, polar_skip=True):
+def alm2map_s0(alm, nside, lmax, polar_skip=True, precision="fp64", max_group=2):
     alm = torch.as_tensor(alm)
     dev = alm.device if alm.is_cuda else torch.device("cuda")
     alm = alm.to(device=dev, dtype=torch.complex128).contiguous()
-    K = alm.shape[0]
     lmax = int(lmax)
     L = lmax + 1
-    if alm.ndim != 2 or K not in (1, 2) or alm.shape[1] != L * (L + 1) // 2:
+    if alm.ndim != 2 or alm.shape[0] < 1 or alm.shape[1] != L * (L + 1) // 2:
         raise ValueError("Input alm has wrong shape")
+    groups = _groups(alm.shape[0], _mixed(precision), max_group)
+    if len(groups) > 1:
+        return torch.cat([alm2map_s0(alm[a:a + k], nside, lmax, polar_skip, precision, max_group)
+                          for a, k in groups])
+    K = alm.shape[0]
     p = plan(nside, lmax, 0, dev, polar_skip)
     nring = 4 * nside - 1
     spec = torch.empty((K, nring, L), dtype=torch.complex128, device=dev)
-    _synthesis_s0(p, alm, spec)
+    _synthesis_s0(p, alm, spec, _mixed(precision))
     npix = 12 * nside * nside
     maps = torch.empty((K, npix), dtype=torch.float64, device=dev)
     _lib.call("gm_ring_fft_inv_s0", _lib.ptr(spec), K, L, spec.stride(0), nside, lmax, _lib.ptr(maps), npix,
@@ -315,7 +337,7 @@ def map2alm_spin(maps, lmax, spin, n_iter=0, polar_skip=True, precision="fp64"):
                    lambda a, s_, mx: _synthesis_s(p, a[0], a[1], s_, mx), fold, spec, _mixed(precision))
 
 
-def alm2map_spin(alm, nside, lmax, spin, polar_skip=True):
+def alm2map_spin(alm, nside, lmax, spin, polar_skip=True, precision="fp64"):
     alm = torch.as_tensor(alm)
     dev = alm.device if alm.is_cuda else torch.device("cuda")
     alm = alm.to(device=dev, dtype=torch.complex128).contiguous()
@@ -327,7 +349,7 @@ def alm2map_spin(alm, nside, lmax, spin, polar_skip=True):
     W = 2 * L - 1
     nring = 4 * nside - 1
     spec = torch.empty((nring, W), dtype=torch.complex128, device=dev)
-    _synthesis_s(p, alm[0], alm[1], spec)
+    _synthesis_s(p, alm[0], alm[1], spec, _mixed(precision))
     npix = 12 * nside * nside
     maps = torch.empty((2, npix), dtype=torch.float64, device=dev)
     _lib.call("gm_ring_fft_inv_s", _lib.ptr(spec), W, nside, lmax, _lib.ptr(maps[0]), _lib.ptr(maps[1]),
@@ -341,7 +363,7 @@ def map2alm(maps, spin, lmax, n_iter=0, polar_skip=True, precision="fp64"):
     return map2alm_spin(maps, lmax, spin, n_iter, polar_skip, precision)
 
 
-def alm2map(alm, spin, nside, lmax, polar_skip=True):
+def alm2map(alm, spin, nside, lmax, polar_skip=True, precision="fp64"):
     if int(spin) == 0:
-        return alm2map_s0(alm, nside, lmax, polar_skip)
-    return alm2map_spin(alm, nside, lmax, spin, polar_skip)
+        return alm2map_s0(alm, nside, lmax, polar_skip, precision)
+    return alm2map_spin(alm, nside, lmax, spin, polar_skip, precision)

@@ -105,7 +105,8 @@ int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const 
                      const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
                      void* stream);
 
-/* Synthesis coefficients: coef[base(m)+l-m][map] = alm[map][idx(l,m)] P_lm (zero padded). */
+/* Synthesis coefficients: coef[base(m)+l-m][map] = alm[map][idx(l,m)] P_lm (zero padded);
+ * nmaps 1, 2, 4 or 8 (4 and 8 feed the batched mixed synthesis). */
 int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_stride, const double* P,
                    int64_t lmax, double* coef, void* stream);
 
@@ -187,8 +188,9 @@ int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
  * block reductions run in FP32 (spectra read and partial sums / spectra written in FP64),
  * ~1e-7 relative.  No counterpart in the reference, whose march is FP32 throughout
  * (gmaster/_native/cuda/march_v2.cu:155-407). */
-/* Chunk counts and finishes of the mixed analyses (their ring chunks differ from FP64's). */
-int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin);
+/* Chunk counts and finishes of the mixed analyses (their ring chunks differ from FP64's,
+ * and for spin 0 depend on the number of maps sharing the recurrence: 1, 2, 4 or 8). */
+int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin, int64_t nmaps);
 int gm_lat_finish_s0_mixed(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
                            const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode, void* stream);
 int gm_lat_finish_s_mixed(const double* partial, int64_t nside, const int32_t* nactive, const double* P, int64_t lmax,
