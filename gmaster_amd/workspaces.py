@@ -196,7 +196,8 @@ class NmtWorkspace:
         self.mcm_binned = None
         self._bpws = None
         if fname is not None:
-            raise NotImplementedError("workspace FITS I/O is not provided by this engine")
+            self.read_from(fname)
+            return
         if fl1 is not None or fl2 is not None or bins is not None:
             self.compute_coupling_matrix(fl1, fl2, bins, is_teb=is_teb, l_toeplitz=l_toeplitz, l_exact=l_exact,
                                          dl_band=dl_band, normalization=normalization)
@@ -204,6 +205,101 @@ class NmtWorkspace:
     @classmethod
     def from_fields(cls, fl1, fl2, bins, **kwargs):
         return cls(fl1, fl2, bins, **kwargs)
+
+    @classmethod
+    def from_file(cls, fname):
+        """Workspace read from a NaMaster-compatible FITS file."""
+        return cls(fname=fname)
+
+    @property
+    def mcm(self):
+        """Dense coupling matrix, ``(ncls (lmax+1))^2`` in ``l * ncls + c`` ordering."""
+        return self.get_coupling_matrix()
+
+    # ------------------------------------------------------------------ FITS I/O
+    def write_to(self, fname):
+        """Write the workspace as a NaMaster-compatible FITS file (``gmaster/workspaces.py:2211-2255``):
+        primary image WSP_PRIMARY (dense MCM + keywords), tables BEAMS, PCL_MASKS, BANDPOWERS."""
+        from . import fits
+
+        tl, te, db = self.toeplitz if getattr(self, "toeplitz", None) else (-1, -1, -1)
+        header = {"EXTNAME": "WSP_PRIMARY", "LMAX": self.lmax, "LMAX_MASK": self.lmax_mask, "IS_TEB": False,
+                  "NCLS": self.ncls, "NORM_TYPE": self.norm_type, "WAWB": float(self.wawb), "SPIN1": self.spin1,
+                  "SPIN2": self.spin2, "ANISO1": False, "ANISO2": False, "PURE_E1": False, "PURE_B1": False,
+                  "PURE_E2": False, "PURE_B2": False, "L_TOEPLITZ": tl, "L_EXACT": te, "DL_BAND": db}
+        L = self.lmax + 1
+        ell = np.arange(L, dtype=np.int32)
+        b = self.bins
+        bands = np.full(L, -1, dtype=np.int32)
+        weights = np.zeros(L)
+        f_ell = np.zeros(L)
+        bands[b._ells_np] = b._bpws_np
+        weights[b._ells_np] = b._weights_np
+        f_ell[b._ells_np] = b._f_ell_np
+        pcl = self.pcl_mask.detach().cpu().numpy().astype(np.float64)
+        fits.write(fname, self.get_coupling_matrix().cpu().numpy(), header, [
+            ("BEAMS", [("L", ell), ("BEAM1", self.beam1.cpu().numpy().astype(np.float64)),
+                       ("BEAM2", self.beam2.cpu().numpy().astype(np.float64))], {}),
+            ("PCL_MASKS", [("L", np.arange(len(pcl), dtype=np.int32)), ("PCL_MASKS", pcl)], {}),
+            ("BANDPOWERS", [("BAND", bands), ("ELLS", ell), ("WEIGHTS", weights), ("F_ELL", f_ell)],
+             {"ELL_MAX": self.lmax, "N_BANDS": b.get_n_bands()}),
+        ])
+
+    def read_from(self, fname):
+        """Read a NaMaster-compatible FITS workspace (``gmaster/workspaces.py:2162-2209``)."""
+        from . import fits
+        from .bins import NmtBin
+
+        hdus = fits.read(fname)
+        h, dense = fits.find(hdus, "WSP_PRIMARY")
+        if bool(h.get("IS_TEB", False)):
+            raise NotImplementedError("joint TEB workspaces are not supported by this engine")
+        if any(bool(h.get(k, False)) for k in ("PURE_E1", "PURE_B1", "PURE_E2", "PURE_B2", "ANISO1", "ANISO2")):
+            raise NotImplementedError("purified or anisotropic workspaces are not supported by this engine")
+        self.lmax = int(h["LMAX"])
+        self.lmax_mask = int(h["LMAX_MASK"])
+        self.ncls = int(h["NCLS"])
+        self.norm_type = int(h.get("NORM_TYPE", 0))
+        self.normalization = "MASTER" if self.norm_type == 0 else "FKP"
+        self.wawb = float(h.get("WAWB", 0.0))
+        self.spin1 = int(h.get("SPIN1", 0 if self.ncls == 1 else 2))
+        self.spin2 = int(h.get("SPIN2", 0 if self.ncls == 1 else 2))
+        self.nmaps1 = 1 if self.spin1 == 0 else 2
+        self.nmaps2 = 1 if self.spin2 == 0 else 2
+        lt = int(h.get("L_TOEPLITZ", -1))
+        self.toeplitz = (lt, int(h.get("L_EXACT", -1)), int(h.get("DL_BAND", -1))) if lt > 0 else None
+        self.coupling_info = {"form": "file"}
+        L = self.lmax + 1
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if dense is None or dense.shape != (L * self.ncls, L * self.ncls):
+            raise ValueError("WSP_PRIMARY does not hold a coupling matrix of the stated size")
+        self.layout = _layout(self.ncls, self.spin1, self.spin2)
+        # the distinct blocks, read back from their first slot in the dense matrix
+        d4 = torch.as_tensor(dense, dtype=torch.float64, device=dev).reshape(L, self.ncls, L, self.ncls)
+        self.blocks = {}
+        for name, (c1, c2), sign in self.layout:
+            if name not in self.blocks:
+                self.blocks[name] = (sign * d4[:, c1, :, c2]).contiguous()
+        for name in ("00", "02", "22p", "22m"):
+            self.blocks.setdefault(name, torch.zeros((L, L), dtype=torch.float64, device=dev))
+        _, beams = fits.find(hdus, "BEAMS")
+        if "BEAMS" in beams:
+            common = np.sqrt(beams["BEAMS"].astype(np.float64))
+            b1 = b2 = common
+        else:
+            b1, b2 = beams["BEAM1"].astype(np.float64), beams["BEAM2"].astype(np.float64)
+        self.beam1 = torch.as_tensor(b1, device=dev)
+        self.beam2 = torch.as_tensor(b2, device=dev)
+        _, pcl = fits.find(hdus, "PCL_MASKS")
+        self.pcl_mask = torch.as_tensor(pcl["PCL_MASKS"].astype(np.float64), device=dev)
+        name = "BINS" if any(str(hh.get("EXTNAME", "")).strip() == "BINS" for hh, _ in hdus) else "BANDPOWERS"
+        bh, bd = fits.find(hdus, name)
+        keep = bd["BAND"] >= 0
+        self.bins = NmtBin(bpws=bd["BAND"][keep].astype(np.int64), ells=bd["ELLS"][keep].astype(np.int64),
+                           lmax=int(bh["ELL_MAX"]), weights=bd["WEIGHTS"][keep].astype(np.float64),
+                           f_ell=bd["F_ELL"][keep].astype(np.float64))
+        self.nbands = self.bins.get_n_bands()
+        self._postprocess()
 
     def compute_coupling_matrix(self, fl1, fl2, bins, is_teb=False, l_toeplitz=-1, l_exact=-1, dl_band=-1,
                                 normalization="MASTER"):

@@ -247,3 +247,30 @@ def test_workspace_toeplitz_blocks_and_bandpowers():
     assert np.all(np.isfinite(bp_t)) and rel > 0
     with pytest.raises(ValueError, match="smaller than"):
         nmt.NmtWorkspace.from_fields(f0, f2, b, l_toeplitz=47, l_exact=10, dl_band=8)
+
+
+@pytest.mark.parametrize("spins", [(0, 0), (0, 2), (2, 2)])
+def test_workspace_fits_round_trip(tmp_path, spins):
+    """write_to / from_file keep the coupling matrix bit for bit and the decoupled
+    bandpowers (the reference's own check, tests/test_workspaces.py:684-716, without
+    pymaster)."""
+    import gmaster as nmt
+
+    fx = np.load(os.path.join(GOLDEN, "pipeline_nside16.npz"))
+    mask = torch.as_tensor(fx["mask"], device="cuda")
+    tqu = torch.as_tensor(fx["tqu"], device="cuda")
+    fields = {0: nmt.NmtField(mask, tqu[0:1], n_iter=3), 2: nmt.NmtField(mask, tqu[1:3], n_iter=3)}
+    f1, f2 = fields[spins[0]], fields[spins[1]]
+    b = nmt.NmtBin.from_nside_linear(16, 4)
+    w = nmt.NmtWorkspace.from_fields(f1, f2, b)
+    w.update_beams(np.linspace(1, 0.5, 48), np.linspace(1, 0.7, 48))
+    path = str(tmp_path / "w.fits")
+    w.write_to(path)
+    r = nmt.NmtWorkspace.from_file(path)
+    assert torch.equal(r.mcm.cpu(), w.mcm.cpu())
+    assert (r.ncls, r.spin1, r.spin2, r.lmax) == (w.ncls, w.spin1, w.spin2, w.lmax)
+    cl = nmt.compute_coupled_cell(f1, f2)
+    np.testing.assert_allclose(np.asarray(r.decouple_cell(cl)), np.asarray(w.decouple_cell(cl)), rtol=1e-13,
+                               atol=0)
+    np.testing.assert_allclose(np.asarray(r.get_bandpower_windows()), np.asarray(w.get_bandpower_windows()),
+                               rtol=1e-12, atol=1e-15)
