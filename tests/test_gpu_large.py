@@ -1,0 +1,110 @@
+"""The Legendre kernels in their large-geometry configurations (4-wave workgroups, 5-8
+rings per lane, several ring chunks) against the C/OpenMP oracle on a sample of orders,
+FP64 and mixed.  These are the configurations the Nside-4096 benchmark runs; the numpy
+oracle is too slow here, the C oracle (an independent natural-form recurrence with its own
+extended exponent, itself pinned against the numpy oracle) is not."""
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sht as osht
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def cb():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle", "c")], check=True)
+    from oracle.c import cpu_baseline
+
+    return cpu_baseline
+
+
+def _orders(L):
+    if L > 4000:  # the benchmark size: fewer orders keep the CPU side to seconds
+        return np.unique(np.array([0, 2, 999, L // 2, L - 300, L - 1]))
+    return np.unique(np.array([0, 1, 2, 3, 7, 64, L // 4, L // 2, (3 * L) // 4, L - 40, L - 2, L - 1]))
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("nside", [512, 1024, 4096])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_spin0_analysis_large_geometry(cb, nside, mixed):
+    from gmaster_amd import _lib, sht
+
+    print(f"spin-0 large geometry nside={nside} mixed={mixed}", flush=True)
+    lmax = 3 * nside - 1
+    L = lmax + 1
+    npix = 12 * nside * nside
+    nring = 4 * nside - 1
+    K = 1 if nside > 2048 else 2  # one map at the benchmark size bounds host memory
+    maps = torch.randn((K, npix), dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(nside))
+    spec = torch.empty((K, nring, L), dtype=torch.complex128, device="cuda")
+    _lib.call("gm_ring_fft_fwd_s0", _lib.ptr(maps), K, npix, nside, lmax, _lib.ptr(spec), L, spec.stride(0),
+              _lib.stream_ptr("cuda"))
+    p = sht.plan(nside, lmax, 0, "cuda")
+    nelem = L * (L + 1) // 2
+    alm = torch.empty((K, nelem), dtype=torch.complex128, device="cuda")
+    sht._analysis_s0(p, spec, alm, 0, mixed)
+    orders = _orders(L)
+    alm = alm.cpu().numpy()
+    sp = spec[:, :, torch.as_tensor(orders, device="cuda")].cpu().numpy()
+    del spec
+    ai = osht.AlmInfo(lmax)
+    for k in range(K):
+        G = np.zeros((nring, 2 * L - 1), dtype=np.complex128)
+        G[:, L - 1 + orders] = sp[k]
+        out = cb.lat_ana_s(nside, lmax, 0, G, orders)
+        for n, m in enumerate(orders):
+            ref = out[n, m:, 0] + 1j * out[n, m:, 1]
+            got = alm[k][ai.mstart[m] + np.arange(m, L)]
+            e = _rel(got, ref)
+            assert e < (2e-6 if mixed else 1e-10), (k, m, e)
+    print("  ok", flush=True)
+
+
+@pytest.mark.parametrize("nside", [256, 1024, 4096])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_spin2_analysis_large_geometry(cb, nside, mixed):
+    from gmaster_amd import _lib, sht
+
+    print(f"spin-2 large geometry nside={nside} mixed={mixed}", flush=True)
+    lmax = 3 * nside - 1
+    L = lmax + 1
+    W = 2 * L - 1
+    npix = 12 * nside * nside
+    nring = 4 * nside - 1
+    qu = torch.randn((2, npix), dtype=torch.float64, device="cuda", generator=torch.Generator("cuda").manual_seed(nside + 2))
+    spec = torch.empty((nring, W), dtype=torch.complex128, device="cuda")
+    _lib.call("gm_ring_fft_fwd_s", _lib.ptr(qu[0]), _lib.ptr(qu[1]), nside, lmax, _lib.ptr(spec), W,
+              _lib.stream_ptr("cuda"))
+    p = sht.plan(nside, lmax, 2, "cuda")
+    nelem = L * (L + 1) // 2
+    alm = torch.empty((2, nelem), dtype=torch.complex128, device="cuda")
+    sht._analysis_s(p, spec, alm[0], alm[1], 0, mixed)
+    alm = alm.cpu().numpy()
+    orders = _orders(L)
+    out = cb.lat_ana_s(nside, lmax, 2, spec.cpu().numpy(), orders)
+    ap = np.zeros((L, W), dtype=np.complex128)
+    ell = np.arange(L)
+    for n, m in enumerate(orders):
+        ap[:, L - 1 + m] = out[n, :, 0] + 1j * out[n, :, 1]
+        if m:
+            ap[:, L - 1 - m] = (-1.0) ** (ell + 2) * (out[n, :, 2] + 1j * out[n, :, 3])
+    ai = osht.AlmInfo(lmax)
+    eb = osht.plus_to_eb(ap, ai, 2)
+    for n, m in enumerate(orders):
+        idx = ai.mstart[m] + np.arange(m, L)
+        for c in range(2):
+            e = _rel(alm[c][idx], eb[c][idx])
+            assert e < (2e-6 if mixed else 1e-10), (c, m, e)
+    print("  ok", flush=True)
