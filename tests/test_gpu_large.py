@@ -108,3 +108,29 @@ def test_spin2_analysis_large_geometry(cb, nside, mixed):
             e = _rel(alm[c][idx], eb[c][idx])
             assert e < (2e-6 if mixed else 1e-10), (c, m, e)
     print("  ok", flush=True)
+
+
+@pytest.mark.parametrize("lmax", [3071, 12287])
+def test_coupling_rows_large_lmax(cb, lmax):
+    """The 3j coupling kernel at the benchmark's lmax against the C oracle's independent
+    l3 recursion on a sample of rows (benchmark-like window: a decaying mask spectrum)."""
+    from gmaster_amd.workspaces import clear_coupling_cache, coupling_blocks
+
+    print(f"coupling rows lmax={lmax}", flush=True)
+    L = lmax + 1
+    ell = np.arange(L)
+    window = 4 * np.pi * 0.9 / (1.0 + ell) ** 2.5
+    clear_coupling_cache()
+    blocks = coupling_blocks(torch.as_tensor(window, device="cuda"), lmax)
+    rows = np.array([0, 2, 3, 17, L // 3, L // 2, L - 2, L - 1], dtype=np.int32)
+    raw = cb.coupling_rows(lmax, window / (4 * np.pi), rows)  # (rows, L, 4), columns l2 >= l1
+    col = 2 * ell + 1
+    for k, name in enumerate(("00", "02", "22p", "22m")):
+        got = blocks[name][torch.as_tensor(rows.astype(np.int64), device="cuda")].cpu().numpy()
+        scale = np.max(np.abs(got))
+        for n, l1 in enumerate(rows):
+            ref = raw[n, l1:, k] * col[l1:]
+            err = np.max(np.abs(got[n, l1:] - ref)) / scale
+            assert err < 1e-12, (name, l1, err)
+    clear_coupling_cache()
+    print("  ok", flush=True)
