@@ -176,9 +176,17 @@ def _coupled_cell(alm1, alm2, lmax):
 
 
 def _layout(ncls, spin1, spin2):
-    """``[(block name, (row channel, column channel), sign)]`` (``gmaster/workspaces.py:1851-1903``)."""
+    """``[(block name, (row channel, column channel), sign)]`` (``gmaster/workspaces.py:1851-1903``).
+    ncls 7 is the joint TEB layout: TT, then TE/TB with sign (-1)^spin2, then the spin x spin
+    slots at offset 3 with the M+ / M- signs of spin_sign = 1."""
     if ncls == 1:
         return [("00", (0, 0), 1.0)]
+    if ncls == 7:
+        out = [("00", (0, 0), 1.0)]
+        out += [("02", (c, c), float((-1) ** spin2)) for c in (1, 2)]
+        out += [("22p", (3 + c, 3 + c), 1.0) for c in range(4)]
+        out += [("22m", (3 + c, 6 - c), -1.0 if c in (1, 2) else 1.0) for c in range(4)]
+        return out
     sign = float((-1) ** (spin1 + spin2))
     if ncls == 2:
         return [("02", (0, 0), sign), ("02", (1, 1), sign)]
@@ -223,7 +231,8 @@ class NmtWorkspace:
         from . import fits
 
         tl, te, db = self.toeplitz if getattr(self, "toeplitz", None) else (-1, -1, -1)
-        header = {"EXTNAME": "WSP_PRIMARY", "LMAX": self.lmax, "LMAX_MASK": self.lmax_mask, "IS_TEB": False,
+        header = {"EXTNAME": "WSP_PRIMARY", "LMAX": self.lmax, "LMAX_MASK": self.lmax_mask,
+                  "IS_TEB": bool(getattr(self, "is_teb", False)),
                   "NCLS": self.ncls, "NORM_TYPE": self.norm_type, "WAWB": float(self.wawb), "SPIN1": self.spin1,
                   "SPIN2": self.spin2, "ANISO1": False, "ANISO2": False, "PURE_E1": False, "PURE_B1": False,
                   "PURE_E2": False, "PURE_B2": False, "L_TOEPLITZ": tl, "L_EXACT": te, "DL_BAND": db}
@@ -252,8 +261,7 @@ class NmtWorkspace:
 
         hdus = fits.read(fname)
         h, dense = fits.find(hdus, "WSP_PRIMARY")
-        if bool(h.get("IS_TEB", False)):
-            raise NotImplementedError("joint TEB workspaces are not supported by this engine")
+        self.is_teb = bool(h.get("IS_TEB", False))
         if any(bool(h.get(k, False)) for k in ("PURE_E1", "PURE_B1", "PURE_E2", "PURE_B2", "ANISO1", "ANISO2")):
             raise NotImplementedError("purified or anisotropic workspaces are not supported by this engine")
         self.lmax = int(h["LMAX"])
@@ -262,7 +270,7 @@ class NmtWorkspace:
         self.norm_type = int(h.get("NORM_TYPE", 0))
         self.normalization = "MASTER" if self.norm_type == 0 else "FKP"
         self.wawb = float(h.get("WAWB", 0.0))
-        self.spin1 = int(h.get("SPIN1", 0 if self.ncls == 1 else 2))
+        self.spin1 = int(h.get("SPIN1", 0 if self.ncls in (1, 7) else 2))
         self.spin2 = int(h.get("SPIN2", 0 if self.ncls == 1 else 2))
         self.nmaps1 = 1 if self.spin1 == 0 else 2
         self.nmaps2 = 1 if self.spin2 == 0 else 2
@@ -307,8 +315,8 @@ class NmtWorkspace:
             raise ValueError("Fields have incompatible pixelizations")
         if fl1.ainfo.lmax != bins.lmax:
             raise ValueError(f"Maximum multipoles in bins ({bins.lmax}) and fields ({fl1.ainfo.lmax}) are not the same")
-        if is_teb:
-            raise NotImplementedError("joint TEB workspaces are not supported by this engine")
+        if is_teb and (fl1.spin != 0 or fl2.spin == 0):
+            raise ValueError("If is_teb=True, fl1 must be spin-0 and fl2 non-zero spin")
         toeplitz_sanity(l_toeplitz, l_exact, dl_band, fl1.ainfo.lmax, (fl1, fl2))
         if normalization not in ("MASTER", "FKP"):
             raise ValueError(f"Unknown normalization type {normalization}. Allowed options are 'MASTER' and 'FKP'")
@@ -318,7 +326,8 @@ class NmtWorkspace:
         self.spin1, self.spin2 = fl1.spin, fl2.spin
         self.nmaps1 = 1 if self.spin1 == 0 else 2
         self.nmaps2 = 1 if self.spin2 == 0 else 2
-        self.ncls = self.nmaps1 * self.nmaps2
+        self.is_teb = bool(is_teb)
+        self.ncls = 7 if self.is_teb else self.nmaps1 * self.nmaps2
         self.lmax = fl1.ainfo.lmax
         self.lmax_mask = fl1.ainfo_mask.lmax
         self.beam1 = fl1.beam

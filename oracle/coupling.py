@@ -112,3 +112,20 @@ def assemble_mcm(blocks, spin1, spin2):
     for c in range(4):
         out[:, c, :, 3 - c] += (-sign if c in (1, 2) else sign) * M22m
     return out.reshape(L * ncls, L * ncls), ncls
+
+
+def assemble_mcm_teb(blocks, spin2=2):
+    """Dense joint T/E/B MCM (ncls 7: TT, TE, TB, EE, EB, BE, BB) in ``l * 7 + c`` ordering
+    (``gmaster/workspaces.py:1871-1903`` with ``is_teb``): 00 at (0, 0), the 0-s block at
+    (1, 1) and (2, 2) with sign (-1)^spin2, M+ on the spin x spin diagonal and M- on its
+    anti-diagonal (signs +, -, -, +) at offset 3."""
+    M00, M02, M22p, M22m = blocks
+    L = M00.shape[0]
+    out = np.zeros((L, 7, L, 7))
+    out[:, 0, :, 0] = M00
+    for c in (1, 2):
+        out[:, c, :, c] = (-1) ** spin2 * M02
+    for c in range(4):
+        out[:, 3 + c, :, 3 + c] += M22p
+        out[:, 3 + c, :, 6 - c] += (-1.0 if c in (1, 2) else 1.0) * M22m
+    return out.reshape(L * 7, L * 7)

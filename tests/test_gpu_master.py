@@ -274,3 +274,35 @@ def test_workspace_fits_round_trip(tmp_path, spins):
                                atol=0)
     np.testing.assert_allclose(np.asarray(r.get_bandpower_windows()), np.asarray(w.get_bandpower_windows()),
                                rtol=1e-12, atol=1e-15)
+
+
+def test_teb_workspace_matches_oracle_layout_and_separate_workspaces(tmp_path):
+    """is_teb: the dense MCM is the oracle's joint layout of the same blocks, and with one
+    mask the 7 decoupled spectra equal the 00, 02 and 22 workspaces' (a block-diagonal
+    system), also after a FITS round trip."""
+    import gmaster as nmt
+    from oracle.coupling import assemble_mcm_teb
+
+    fx = np.load(os.path.join(GOLDEN, "pipeline_nside16.npz"))
+    mask = torch.as_tensor(fx["mask"], device="cuda")
+    tqu = torch.as_tensor(fx["tqu"], device="cuda")
+    f0 = nmt.NmtField(mask, tqu[0:1], n_iter=3)
+    f2 = nmt.NmtField(mask, tqu[1:3], n_iter=3)
+    b = nmt.NmtBin.from_nside_linear(16, 4)
+    with pytest.raises(ValueError, match="is_teb"):
+        nmt.NmtWorkspace.from_fields(f2, f0, b, is_teb=True)
+    wt = nmt.NmtWorkspace.from_fields(f0, f2, b, is_teb=True)
+    assert wt.ncls == 7
+    blocks = [wt.blocks[k].cpu().numpy() for k in ("00", "02", "22p", "22m")]
+    np.testing.assert_array_equal(np.asarray(wt.get_coupling_matrix().cpu()), assemble_mcm_teb(blocks))
+    cl = torch.cat([nmt.compute_coupled_cell(f0, f0), nmt.compute_coupled_cell(f0, f2),
+                    nmt.compute_coupled_cell(f2, f2)])
+    bp = np.asarray(wt.decouple_cell(cl))
+    sep = [np.asarray(nmt.NmtWorkspace.from_fields(a, c, b).decouple_cell(nmt.compute_coupled_cell(a, c)))
+           for a, c in ((f0, f0), (f0, f2), (f2, f2))]
+    np.testing.assert_allclose(bp, np.concatenate(sep), rtol=1e-10, atol=1e-15 * np.abs(bp).max())
+    path = str(tmp_path / "teb.fits")
+    wt.write_to(path)
+    r = nmt.NmtWorkspace.from_file(path)
+    assert r.is_teb and r.ncls == 7
+    np.testing.assert_allclose(np.asarray(r.decouple_cell(cl)), bp, rtol=1e-13, atol=0)

@@ -115,3 +115,25 @@ def test_pipeline_fixture_is_reproducible():
     res = master.pipeline_tp(fx["mask"], fx["tqu"], 16, 47, 8, 3)
     for key in ("bp_00", "bp_02", "bp_22", "pcl_mask"):
         assert np.max(np.abs(res[key] - fx[key])) <= 1e-12 * np.max(np.abs(fx[key]))
+
+
+def test_teb_assembly_is_the_direct_sum_of_the_separate_layouts():
+    """The joint layout holds the 00, 0-2 and 2-2 matrices of the separate workspaces as
+    diagonal blocks (channels 0, 1-2, 3-6) and nothing else."""
+    from oracle.coupling import assemble_mcm, assemble_mcm_teb
+
+    rng = np.random.default_rng(7)
+    L = 9
+    blocks = [rng.normal(size=(L, L)) for _ in range(4)]
+    teb = assemble_mcm_teb(blocks).reshape(L, 7, L, 7)
+    m00, _ = assemble_mcm(blocks, 0, 0)
+    m02, _ = assemble_mcm(blocks, 0, 2)
+    m22, _ = assemble_mcm(blocks, 2, 2)
+    np.testing.assert_array_equal(teb[:, 0, :, 0], m00)
+    np.testing.assert_array_equal(teb[:, 1:3, :, 1:3].reshape(2 * L, 2 * L), m02)
+    np.testing.assert_array_equal(teb[:, 3:, :, 3:].reshape(4 * L, 4 * L), m22)
+    off = teb.copy()
+    off[:, 0, :, 0] = 0
+    off[:, 1:3, :, 1:3] = 0
+    off[:, 3:, :, 3:] = 0
+    assert not off.any()
