@@ -24,6 +24,9 @@ COMMON = [
     f"--offload-arch={ARCH}",
     # no packed FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: with AGPR accumulators the register allocator
+    # rotates the Legendre kernels' parity accumulators through copies every iteration
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I", os.path.join(HERE, "..", "include"),
 ]
 

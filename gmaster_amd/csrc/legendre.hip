@@ -517,6 +517,240 @@ __global__ __launch_bounds__(64 * NW) void k_syn_s(const double2* __restrict__ c
     }
 }
 
+// ------------------------------------------------------------------ matrix-core kernels
+// Mixed-precision synthesis on the matrix unit: the recurrence stays in FP64 on the vector
+// ALU and the contraction runs as FP32 MFMA (v_mfma_f32_4x4x1f32: 16 independent 4x4
+// outer products per wave, block b = lanes 4b..4b+3; A row r and B column c come from
+// lanes 4b+r and 4b+c, and the result for (r, c) lands in register r of lane 4b+c).
+// A = lambda of the lane's own ring at one degree, B = the degree's coefficient of channel
+// (lane & 3), so one instruction adds a degree to a 4 ring x 4 channel tile per block:
+// 8 cycles where four FP32 FMAs take 16-18.  On MI355X the matrix unit and the vector ALU
+// of a SIMD do not run concurrently (measured: 8 MFMA + 16 FP64 FMA take the sum of both
+// when grouped, and a third more when interleaved), so each tile runs the vector work for
+// all its degrees first and then the matrix work, with scheduling barriers keeping the two
+// apart.  The analysis stays on the vector ALU: there the rings are the contraction index,
+// and bringing a ring's degrees across the quad costs as many matrix cycles as it saves.
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0); }
+
+// Spin-0 synthesis on the matrix unit: one MFMA per degree and ring set and channel group.
+template <int K, int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_syn_s0_mx(const double2* __restrict__ coef, int nside, int L,
+                                                       const double* __restrict__ alpha,
+                                                       const double* __restrict__ smant,
+                                                       const int* __restrict__ sexp, const int* __restrict__ nact,
+                                                       double2* __restrict__ spec, int64_t ld, int64_t smap) {
+    constexpr int C = 2 * K;
+    constexpr int NG = (C + 3) / 4;
+    constexpr int T = 64 * NW;
+    constexpr int CH = T * R;
+    const int c = blockIdx.x, m = blockIdx.y;
+    const int na = nact[m];
+    const int NF = 2 * nside;
+    const int q4 = threadIdx.x & 3;
+    const int qbase = threadIdx.x & ~3;
+
+    double x[R], mu0[R], mu1[R];
+    int kx[R];
+    f4 acc[2][R][NG];  // [parity E, O]
+    const double sm = smant[m];
+    const int se = sexp[m];
+    bool anyact = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        mu1[t] = 0;
+        kx[t] = 0;
+        x[t] = 0;
+        mu0[t] = 0;
+        if (j < na) {
+            const Ring g = ring_info(nside, 2 * nside - 1 - j);
+            x[t] = g.z;
+            seed_lane<double>(sm, se, m, m, g.shalf, g.chalf, mu0[t], kx[t]);
+            anyact = true;
+        }
+#pragma unroll
+        for (int q = 0; q < NG; ++q) acc[0][t][q] = acc[1][t][q] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    if (__any(anyact)) {
+        bool anyinact = false;
+#pragma unroll
+        for (int t = 0; t < R; ++t) anyinact = anyinact || (kx[t] < 0);
+        anyinact = __any(anyinact);
+        const double* al = alpha + table_base(m, L) - m;
+        // coefficient of channel ch at degree l: cf[l * C + ch] (zero padded past lmax)
+        const double* cf = reinterpret_cast<const double*>(coef + (table_base(m, L) - m) * K);
+        double an[4], cn[4][NG];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            an[i] = al[m + i + 1];
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                const int ch = 4 * q + q4;
+                cn[i][q] = ch < C ? cf[(int64_t)(m + i) * C + ch] : 0.0;
+            }
+        }
+        for (int l0 = m; l0 < L; l0 += 4) {
+            double a[4];
+            float cb[4][NG];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = an[i];
+                an[i] = al[l0 + 4 + i + 1];
+#pragma unroll
+                for (int q = 0; q < NG; ++q) {
+                    const int ch = 4 * q + q4;
+                    cb[i][q] = (float)cn[i][q];
+                    cn[i][q] = ch < C ? cf[(int64_t)(l0 + 4 + i) * C + ch] : 0.0;  // next tile
+                }
+            }
+            // vector burst (recurrence) then matrix burst: the two units do not overlap
+            // and every switch between them costs issue cycles
+            float cur[R][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) cur[t][i] = (float)((i & 1) ? mu1[t] : mu0[t]);
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    if (i & 1) mu0[t] = fma(x[t] * a[i], mu1[t], -mu0[t]);
+                    else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int t = 0; t < R; ++t)
+#pragma unroll
+                    for (int q = 0; q < NG; ++q) acc[i & 1][t][q] = mfma4(cur[t][i], cb[i][q], acc[i & 1][t][q]);
+            __builtin_amdgcn_sched_barrier(0);
+            rescale<R, double>(mu0, mu1, kx, anyinact);
+        }
+    }
+    // acc[p][t][q][v] at lane (b, q4): ring qbase + v of set t, channel 4q + q4
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = c * CH + t * T + qbase + v;
+            if (j >= NF) continue;
+            const int r = 2 * nside - 1 - j, rp = 2 * nside - 1 + j;
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                const int ch = 4 * q + q4;
+                if (ch >= C) continue;
+                const int k = ch >> 1, comp = ch & 1;
+                const double ev = acc[0][t][q][v], ov = acc[1][t][q][v];
+                reinterpret_cast<double*>(spec + k * smap + (int64_t)r * ld + m)[comp] = ev + ov;
+                if (j > 0) reinterpret_cast<double*>(spec + k * smap + (int64_t)rp * ld + m)[comp] = ev - ov;
+            }
+        }
+    }
+}
+
+// Spin-s synthesis on the matrix unit.
+template <int R, int NW>
+__global__ __launch_bounds__(64 * NW) void k_syn_s_mx(const double2* __restrict__ coef, int nside, int L, int s,
+                                                      const double* __restrict__ alpha,
+                                                      const double* __restrict__ gamma,
+                                                      const double* __restrict__ smant,
+                                                      const int* __restrict__ sexp, const int* __restrict__ nact,
+                                                      double2* __restrict__ spec, int64_t ld) {
+    constexpr int T = 64 * NW;
+    constexpr int CH = T * R;
+    const int c = blockIdx.x, m = blockIdx.y;
+    const int na = nact[m];
+    const int nring = 4 * nside - 1;
+    const int M = m > s ? m : s;
+    const int q4 = threadIdx.x & 3;
+    const int qbase = threadIdx.x & ~3;
+
+    double x[R], mu0[R], mu1[R];
+    int kx[R];
+    f4 acc[R];
+    const double sm = smant[m];
+    const int se = sexp[m];
+    const int a_exp = m + s, b_exp = m > s ? m - s : s - m;
+    bool anyact = false;
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+        const int j = c * CH + t * T + threadIdx.x;
+        mu1[t] = 0;
+        kx[t] = 0;
+        x[t] = 0;
+        mu0[t] = 0;
+        acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+        if (j < na && M < L) {
+            const Ring g = ring_info(nside, slot_ring(nside, j));
+            x[t] = g.z;
+            seed_lane<double>(sm, se, a_exp, b_exp, g.shalf, g.chalf, mu0[t], kx[t]);
+            anyact = true;
+        }
+    }
+    if (__any(anyact)) {
+        bool anyinact = false;
+#pragma unroll
+        for (int t = 0; t < R; ++t) anyinact = anyinact || (kx[t] < 0);
+        anyinact = __any(anyinact);
+        const double* al = alpha + table_base(m, L) - m;
+        const double* ga = gamma + table_base(m, L) - m;
+        // channel q4 of degree l: (direct re, im, mirror re, im) at cf[4 l + q4]
+        const double* cf = reinterpret_cast<const double*>(coef + (table_base(m, L) - m) * 2);
+        double an[4], gn[4], cn[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            an[i] = al[M + i + 1];
+            gn[i] = ga[M + i + 1];
+            cn[i] = cf[(int64_t)(M + i) * 4 + q4];
+        }
+        for (int l0 = M; l0 < L; l0 += 4) {
+            double a[4], gg[4];
+            float cb[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] = an[i];
+                gg[i] = gn[i];
+                cb[i] = (float)cn[i];
+                an[i] = al[l0 + 4 + i + 1];
+                gn[i] = ga[l0 + 4 + i + 1];
+                cn[i] = cf[(int64_t)(l0 + 4 + i) * 4 + q4];
+            }
+            float cur[R][4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) cur[t][i] = (float)((i & 1) ? mu1[t] : mu0[t]);
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    if (i & 1) mu0[t] = fma(fma(x[t], a[i], gg[i]), mu1[t], -mu0[t]);
+                    else mu1[t] = fma(fma(x[t], a[i], gg[i]), mu0[t], -mu1[t]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int t = 0; t < R; ++t) acc[t] = mfma4(cur[t][i], cb[i], acc[t]);
+            __builtin_amdgcn_sched_barrier(0);
+            rescale<R, double>(mu0, mu1, kx, anyinact);
+        }
+    }
+    // acc[t][v] at lane (b, q4): ring qbase + v of set t, channel q4
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = c * CH + t * T + qbase + v;
+            if (j >= nring) continue;
+            const int r = slot_ring(nside, j);
+            if (q4 < 2) reinterpret_cast<double*>(spec + (int64_t)r * ld + (L - 1) + m)[q4] = acc[t][v];
+            else if (m) reinterpret_cast<double*>(spec + (int64_t)(nring - 1 - r) * ld + (L - 1) - m)[q4 - 2] = acc[t][v];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ tables
 __global__ void k_lat_tables(int L, int s, double* __restrict__ alpha, double* __restrict__ gamma,
                              double* __restrict__ P) {
@@ -661,11 +895,9 @@ __global__ void k_prep_s(const double2* __restrict__ ae, const double2* __restri
 }
 
 // ------------------------------------------------------------------ configs
-// Large geometries use 4-wave workgroups with R rings per lane: R = 6 for spin 0 (two
-// maps' E/O coefficients per ring keep a wave under 256 VGPRs, i.e. two waves per SIMD),
-// R = 8 for spin s.
-// Rings per lane of the large geometries, per precision and pass (FP32-contraction kernels
-// hold half the registers per ring).  Chosen by timing R = 3..8 on MI355X at Nside 4096:
+// Large geometries use 4-wave workgroups with R rings per lane, per precision and pass
+// (the mixed kernels hold half the registers per ring; the mixed syntheses keep their
+// accumulators in MFMA tiles).  Chosen by timing R = 3..10 on MI355X at Nside 4096:
 // occupancy steps make the best R differ between passes.
 #ifndef GM_ANA_R_S0
 #define GM_ANA_R_S0 6
@@ -689,7 +921,7 @@ __global__ void k_prep_s(const double2* __restrict__ ae, const double2* __restri
 #define GM_MIX_ANA_R_S 6
 #endif
 #ifndef GM_MIX_SYN_R_S
-#define GM_MIX_SYN_R_S 5
+#define GM_MIX_SYN_R_S 6
 #endif
 
 // Batched spin-0 maps (K = 4 or 8 sharing one recurrence, mixed precision only) hold 4K
@@ -832,7 +1064,15 @@ int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t n
     double2* sp = reinterpret_cast<double2*>(spec);
     constexpr int RL = large_r(true, mixed, true);
     if (int e = check_config(name, cfg, large_r(true, mixed, true, (int)nmaps))) return e;
-#define GM_SYN0(KK, RR, WW) hipLaunchKernelGGL((k_syn_s0<KK, RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, sp, ld, smap)
+#define GM_SYN0(KK, RR, WW)                                                                                       \
+    do {                                                                                                          \
+        if constexpr (mixed)                                                                                      \
+            hipLaunchKernelGGL((k_syn_s0_mx<KK, RR, WW>), grid, block, 0, st, cf, (int)nside, (int)L, alpha,      \
+                               seed_mant, seed_exp, nactive, sp, ld, smap);                                        \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_syn_s0<KK, RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, alpha,      \
+                               seed_mant, seed_exp, nactive, sp, ld, smap);                                        \
+    } while (0)
     if (nmaps == 1) {
         if (cfg.NW == 1) GM_SYN0(1, 2, 1);
         else if (cfg.NW == 2) GM_SYN0(1, 4, 2);
@@ -900,7 +1140,15 @@ int launch_syn_s(const char* name, const double* coef, int64_t nside, int64_t lm
     double2* sp = reinterpret_cast<double2*>(spec);
     constexpr int RL = large_r(true, std::is_same<F, float>::value, false);
     if (int e = check_config(name, cfg, RL)) return e;
-#define GM_SYNS(RR, WW) hipLaunchKernelGGL((k_syn_s<RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, sp, ld)
+#define GM_SYNS(RR, WW)                                                                                           \
+    do {                                                                                                          \
+        if constexpr (std::is_same<F, float>::value)                                                             \
+            hipLaunchKernelGGL((k_syn_s_mx<RR, WW>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin,       \
+                               alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);                                \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_syn_s<RR, WW, F>), grid, block, 0, st, cf, (int)nside, (int)L, (int)spin,       \
+                               alpha, gamma, seed_mant, seed_exp, nactive, sp, ld);                                \
+    } while (0)
     if (cfg.NW == 1) GM_SYNS(2, 1);
     else if (cfg.NW == 2) GM_SYNS(4, 2);
     else GM_SYNS(RL, 4);

@@ -445,10 +445,21 @@ struct Bucket {
     int N;         // largest N = nphi / 2 in the bucket
 };
 
-// Size buckets: caps with N = 2q in (2^(b-1), 2^b], then the belt.
+// Size buckets: the small caps together (N = 2q <= 512: one launch instead of one per
+// power of two, each of which would hold a handful of workgroups), then caps with
+// N = 2q in (2^(b-1), 2^b], then the belt.
 int make_buckets(int nside, Bucket* out) {
     int nb = 0;
     int q = 1;
+    if (nside - 1 >= 1) {
+        const int qhi = nside - 1 < 256 ? nside - 1 : 256;
+        out[nb].qlo = 1;
+        out[nb].qhi = qhi;
+        out[nb].nblocks = 2 * qhi;
+        out[nb].N = 2 * qhi;
+        ++nb;
+        q = qhi + 1;
+    }
     while (q <= nside - 1) {
         int N = 2 * q;
         int pow2 = 1;
@@ -469,9 +480,11 @@ int make_buckets(int nside, Bucket* out) {
     return nb + 1;
 }
 
+// At least N / 8 threads (the passes' assumption), and at least 256 so that the output
+// loop over up to 2 lmax + 1 phased coefficients of a small ring is not left to one wave.
 inline int threads_for(int N) {
     int t = N / 8;
-    if (t < 64) t = 64;
+    if (t < 256) t = 256;
     t = (t + 63) / 64 * 64;
     if (t > 1024) t = 1024;
     return t;
