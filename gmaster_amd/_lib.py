@@ -53,6 +53,10 @@ SIGNATURES = {
     "gm_toeplitz": [_p, _i64, _i64, _i64, _i64, _p, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
+    "gm_map_dots_work_size": [_i64, _i64, _i64],
+    "gm_map_dots": [_p, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64, _d, _p, _p, _p],
+    "gm_map_deproject": [_p, _i64, _p, _i64, _i64, _p, _p],
+    "gm_alm_filter": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p],
     "gm_table_size": [_i64],
     "gm_lat_nchunks": [_i64, _i64],
     "gm_lat_nchunks_mixed": [_i64, _i64, _i64],
@@ -60,7 +64,7 @@ SIGNATURES = {
     "gm_abi_version": [],
     "gm_last_error": [],
 }
-_RET_I64 = {"gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
+_RET_I64 = {"gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
 
 _lib = None
 

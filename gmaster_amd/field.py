@@ -1,16 +1,19 @@
 """Curved-sky fields (``pymaster.NmtField``), restating ``gmaster/field.py:78-376``.
 
-A field masks its maps and analyses them (``n_iter`` Jacobi steps).  A spin-0 field
-whose mask has the same band limit and ``n_iter`` analyses the masked map and the mask
-in one shared latitudinal pass (``gmaster/field.py:273-289``); other fields compute
-their mask alms lazily (``:348-376``).  Contaminant deprojection, purification and
-anisotropic masks are outside this engine and raise ``NotImplementedError``.
+A field masks its maps, subtracts the best fit of its (masked) contaminant templates
+(``gmaster/field.py:193-247``: covariance and products by ``gm_map_dots``, pseudo-inverse
+on the host, subtraction by ``gm_map_deproject``) and analyses the result (``n_iter``
+Jacobi steps).  A spin-0 field whose mask has the same band limit and ``n_iter`` analyses
+the masked map and the mask in one shared latitudinal pass (``gmaster/field.py:273-289``);
+other fields compute their mask alms lazily (``:348-376``).  Purification and anisotropic
+masks are outside this engine and raise ``NotImplementedError``.
 """
 
 import torch
 
 from . import sht
 from ._array import device_tensor, wrap
+from .deproject import deproject
 from ._config import nmt_params
 from .utils import NmtAlmInfo, NmtMapInfo
 
@@ -45,8 +48,6 @@ class NmtField:
     def __init__(self, mask, maps, *, spin=None, templates=None, beam=None, purify_e=False, purify_b=False,
                  n_iter=None, n_iter_mask=None, tol_pinv=None, wcs=None, lmax=None, lmax_mask=None,
                  masked_on_input=False, lite=False, mask_22=None, mask_12=None):
-        if templates is not None:
-            raise NotImplementedError("contaminant deprojection (templates) is not supported by this engine")
         if purify_e or purify_b:
             raise NotImplementedError("E/B purification is not supported by this engine")
         if mask_22 is not None or mask_12 is not None:
@@ -60,6 +61,7 @@ class NmtField:
         self.alm = self.alm_mask = None
         self.maps = None
         self.n_temp = 0
+        self.temp = self.alm_temp = self.alphas = self.iM = None
         self.Nw = 0.0
         self.Nf = 0.0
 
@@ -84,6 +86,8 @@ class NmtField:
             self.beam = b[: lmax + 1]
 
         if maps is None:
+            if templates is not None:
+                raise ValueError("Templates need the field's maps")
             if spin is None:
                 raise ValueError("Please supply field spin")
             self.spin = int(spin)
@@ -105,6 +109,18 @@ class NmtField:
         self.nmaps = 1 if self.spin == 0 else 2
 
         masked = maps_t if masked_on_input else maps_t * self.mask[None, :]
+        temps = None
+        if templates is not None:
+            temps = device_tensor(templates, device=dev)
+            if temps.ndim != 3 or tuple(temps.shape[1:]) != tuple(maps_t.shape):
+                raise ValueError("Each template must have the same shape as the maps")
+            if not masked_on_input:
+                temps = temps * self.mask[None, None, :]
+            if masked is maps_t:
+                masked = masked.clone()  # the subtraction is in place; keep the caller's maps
+            tol = nmt_params.tol_pinv_default if tol_pinv is None else float(tol_pinv)
+            self.alphas, self.iM = deproject(masked, temps, tol)
+            self.n_temp = len(temps)
         key = _mask_key(self.mask, lmax_mask, self.n_iter_mask)
         self.alm_mask = _MASK_ALMS.get(key)
         if (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
@@ -118,6 +134,14 @@ class NmtField:
             self.alm = sht.map2alm(masked, self.spin, lmax, n_iter=self.n_iter, precision=_precision())
         if not self.lite:
             self.maps = masked
+            if temps is not None:
+                self.temp = temps
+                if self.spin == 0:
+                    self.alm_temp = sht.map2alm_s0(temps[:, 0, :], lmax, n_iter=self.n_iter,
+                                                   precision=_precision())[:, None, :]
+                else:
+                    self.alm_temp = torch.stack([sht.map2alm(t, self.spin, lmax, n_iter=self.n_iter,
+                                                             precision=_precision()) for t in temps])
 
     def is_compatible(self, other, strict=True):
         if strict and self.minfo != other.minfo:
@@ -131,6 +155,11 @@ class NmtField:
         if self.maps is None:
             raise ValueError("Input maps unavailable for this field")
         return wrap(self.maps)
+
+    def get_templates(self):
+        if self.temp is None:
+            raise ValueError("Input templates unavailable for this field")
+        return wrap(self.temp)
 
     def get_alms(self):
         if self.alm is None:

@@ -182,6 +182,29 @@ int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, const int32
 int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
                 const int32_t* band_start, int64_t nbands, const double* w, double* out, void* stream);
 
+/* ---------------------------------------------------------------- contaminant deprojection */
+
+/* Weighted inner products of flattened maps (each vector n doubles, e.g. nmaps x npix):
+ *   out[i*nb + j] = w sum_p a_i[p] b_j[p] v[p mod period]   (v = weight_map; NULL for 1)
+ * with a_i at a + i*lda, b_j at b + j*ldb.  work: >= gm_map_dots_work_size(na, nb, n)
+ * doubles.  The sum is taken in a fixed order (deterministic).  Replaces the template
+ * covariance and template-map products of gmaster/field.py:231-241 and dot_map
+ * (gmaster/utils.py:256-263, weight 4 pi / npix). */
+int64_t gm_map_dots_work_size(int64_t na, int64_t nb, int64_t n);
+int gm_map_dots(const double* a, int64_t na, int64_t lda, const double* b, int64_t nb, int64_t ldb, int64_t n,
+                const double* weight_map, int64_t period, double w, double* out, double* work, void* stream);
+
+/* maps[p] -= sum_i alphas[i] templates[i*ldt + p] for p < n (alphas: ntemp device doubles).
+ * Replaces gmaster/field.py:243-247. */
+int gm_map_deproject(double* maps, int64_t n, const double* templates, int64_t ntemp, int64_t ldt,
+                     const double* alphas, void* stream);
+
+/* out[o] = sum_k cl[k][o][l] alm[k] per healpy-packed element (l, m): cl is nin x nout x
+ * (lmax+1) doubles, alm nin and out nout complex arrays with the given strides (complex
+ * elements); out must not alias alm.  Replaces _filter_alms (gmaster/workspaces.py:2258-2265). */
+int gm_alm_filter(const double* alm, int64_t nin, int64_t alm_stride, const double* cl, int64_t nout,
+                  int64_t lmax, double* out, int64_t out_stride, void* stream);
+
 /* ---- mixed precision: FP32 contraction ---------------------------------------------------
  * Same arguments as the FP64 entries above.  The recurrence, its tables and seeds stay
  * FP64; each step's value is rounded to FP32 once and the contraction over rings and the
