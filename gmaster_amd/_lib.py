@@ -10,7 +10,9 @@ import os
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgmaster_amd.so")
+# GMASTER_AMD_LIB selects another build of the same library (e.g. one compiled with
+# different tuning macros); the default is the in-tree build.
+LIB_PATH = os.environ.get("GMASTER_AMD_LIB") or os.path.join(_HERE, "libgmaster_amd.so")
 
 _i64 = ctypes.c_int64
 _i32 = ctypes.c_int
@@ -26,17 +28,17 @@ SIGNATURES = {
     "gm_ring_fold_residual": [_p, _p, _p, _i64, _i64, _i64, _i64, _i64, _p],
     "gm_ring_fold_residual_c": [_p, _p, _p, _i64, _i64, _i64, _p],
     "gm_lat_tables": [_i64, _i64, _p, _p, _p, _p],
-    "gm_lat_ana_s0": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
-    "gm_lat_ana_s0_mixed": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _p],
-    "gm_lat_finish_s0": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
-    "gm_lat_finish_s0_mixed": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
+    "gm_lat_ana_s0": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "gm_lat_ana_s0_mixed": [_p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "gm_lat_finish_s0": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _i64, _p],
+    "gm_lat_finish_s0_mixed": [_p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _i64, _p],
     "gm_lat_prep_s0": [_p, _i64, _i64, _p, _i64, _p, _p],
     "gm_lat_syn_s0": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
     "gm_lat_syn_s0_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _i64, _i64, _p],
-    "gm_lat_ana_s": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
-    "gm_lat_ana_s_mixed": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
-    "gm_lat_finish_s": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _p],
-    "gm_lat_finish_s_mixed": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _p],
+    "gm_lat_ana_s": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "gm_lat_ana_s_mixed": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "gm_lat_finish_s": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _i64, _p],
+    "gm_lat_finish_s_mixed": [_p, _i64, _p, _p, _i64, _i64, _p, _p, _i32, _i64, _p],
     "gm_lat_prep_s": [_p, _p, _p, _i64, _i64, _p, _p],
     "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_syn_s_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
@@ -60,11 +62,12 @@ SIGNATURES = {
     "gm_table_size": [_i64],
     "gm_lat_nchunks": [_i64, _i64],
     "gm_lat_nchunks_mixed": [_i64, _i64, _i64],
+    "gm_lat_ring_chunks": [_i64, _i64, _i64, _i64],
     "gm_cl_work_size": [_i64, _i64],
     "gm_abi_version": [],
     "gm_last_error": [],
 }
-_RET_I64 = {"gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
+_RET_I64 = {"gm_lat_ring_chunks", "gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
 
 _lib = None
 

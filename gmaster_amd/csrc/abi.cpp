@@ -34,4 +34,4 @@ int check_launch(const char* what) {
 
 extern "C" const char* gm_last_error(void) { return gm::g_last_error; }
 
-extern "C" int gm_abi_version(void) { return 1; }
+extern "C" int gm_abi_version(void) { return 2; }

@@ -121,12 +121,13 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
                                                     const double* __restrict__ smant,
                                                     const int* __restrict__ sexp,
                                                     const int* __restrict__ nact,
-                                                    double* __restrict__ partial, int64_t tsize) {
+                                                    double* __restrict__ partial, int64_t tsize, int chunk0) {
     constexpr int NV = 4 * 2 * K;  // 4 degrees x (re, im) x maps
     constexpr int T = 64 * NW;
     constexpr int CH = T * R;
     __shared__ F lds[2 * NV * T];
-    const int c = blockIdx.x, m = blockIdx.y;
+    // ring chunk chunk0 + blockIdx.x, written to partial slot blockIdx.x
+    const int c = chunk0 + blockIdx.x, m = blockIdx.y;
     const int na = nact[m];
     if (c * CH >= na) return;
 
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s0(const double2* __restrict__ 
     anyinact = __any(anyinact);
 
     const double* al = alpha + table_base(m, L) - m;
-    double* part = partial + ((int64_t)c * tsize + table_base(m, L) - m) * (2 * K);
+    double* part = partial + ((int64_t)blockIdx.x * tsize + table_base(m, L) - m) * (2 * K);
     // mu0 holds degree l (even step), mu1 degree l - 1; each step overwrites the older one,
     // so a block of 4 degrees returns the roles to where they started (no register moves).
     double an[4];
@@ -340,12 +341,13 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
                                                    const double* __restrict__ smant,
                                                    const int* __restrict__ sexp,
                                                    const int* __restrict__ nact,
-                                                   double* __restrict__ partial, int64_t tsize) {
+                                                   double* __restrict__ partial, int64_t tsize, int chunk0) {
     constexpr int NV = 16;  // 4 degrees x 4 channels
     constexpr int T = 64 * NW;
     constexpr int CH = T * R;
     __shared__ F lds[2 * NV * T];
-    const int c = blockIdx.x, m = blockIdx.y;
+    // ring chunk chunk0 + blockIdx.x, written to partial slot blockIdx.x
+    const int c = chunk0 + blockIdx.x, m = blockIdx.y;
     const int na = nact[m];
     if (c * CH >= na) return;
     const int M = m > s ? m : s;
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(64 * NW) void k_ana_s(const double2* __restrict__ s
 
     const double* al = alpha + table_base(m, L) - m;
     const double* ga = gamma + table_base(m, L) - m;
-    double* part = partial + ((int64_t)c * tsize + table_base(m, L) - m) * 4;
+    double* part = partial + ((int64_t)blockIdx.x * tsize + table_base(m, L) - m) * 4;
     double an[4], gn[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -792,16 +794,27 @@ __global__ void k_lat_tables(int L, int s, double* __restrict__ alpha, double* _
 
 // ------------------------------------------------------------------ finish / prep
 // Chunk layout of the analysis kernels: CH rings per workgroup.
+// nchunks ring chunks of CH rings; an analysis pass covers nslots of them (its partial
+// buffer holds nslots chunk slots).  Up to Nside 4096 one pass takes every chunk; above
+// that the passes bound the partial buffer (4 slots of ~9.7 GB at Nside 8192).
+constexpr int kOnePassNside = 4096, kSlotsPerPass = 4;
 struct LatConfig {
-    int R, NW, CH, nchunks;
+    int R, NW, CH, nchunks, nslots;
 };
+
+// mode 0 stores, 1 subtracts, 2 adds; the partial slots hold ring chunks chunk0 .. chunk0 +
+// nchunk - 1, of which those below ceil(nact / CH) were written
+__device__ inline int used_slots(int na, int CH, int chunk0, int nchunk) {
+    int used = (na + CH - 1) / CH - chunk0;
+    if (used < 0) used = 0;
+    return used < nchunk ? used : nchunk;
+}
 
 __global__ void k_finish_s0(const double2* __restrict__ partial, int K, int nchunk, int CH,
                             const int* __restrict__ nact, const double* __restrict__ P, int L, int64_t tsize,
-                            double2* __restrict__ alm, int64_t astride, int mode) {
+                            double2* __restrict__ alm, int64_t astride, int mode, int chunk0) {
     const int m = blockIdx.y;
-    const int used = (nact[m] + CH - 1) / CH;
-    const int nu = used < nchunk ? used : nchunk;
+    const int nu = used_slots(nact[m], CH, chunk0, nchunk);
     const int64_t pb = table_base(m, L) - m;
     const int lmax = L - 1;
     for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l < L; l += gridDim.x * blockDim.x) {
@@ -811,7 +824,7 @@ __global__ void k_finish_s0(const double2* __restrict__ partial, int K, int nchu
             for (int c = 0; c < nu; ++c) s = cadd(s, partial[((int64_t)c * tsize + pb + l) * K + k]);
             s = cscale(s, p);
             double2& a = alm[k * astride + alm_index(l, m, lmax)];
-            a = mode ? csub(a, s) : s;
+            a = mode == 1 ? csub(a, s) : mode == 2 ? cadd(a, s) : s;
         }
     }
 }
@@ -832,10 +845,9 @@ __global__ void k_prep_s0(const double2* __restrict__ alm, int K, int64_t astrid
 
 __global__ void k_finish_s(const double2* __restrict__ partial, int nchunk, int CH, const int* __restrict__ nact,
                            const double* __restrict__ P, int L, int s, int64_t tsize, double2* __restrict__ ae,
-                           double2* __restrict__ ab, int mode) {
+                           double2* __restrict__ ab, int mode, int chunk0) {
     const int m = blockIdx.y;
-    const int used = (nact[m] + CH - 1) / CH;
-    const int nu = used < nchunk ? used : nchunk;
+    const int nu = used_slots(nact[m], CH, chunk0, nchunk);
     const int64_t pb = table_base(m, L) - m;
     const int lmax = L - 1;
     const double sgn_s = (s & 1) ? -1.0 : 1.0;
@@ -860,9 +872,12 @@ __global__ void k_finish_s(const double2* __restrict__ partial, int nchunk, int 
             B = make_double2(-0.5 * (plus.y - minus.y), 0.5 * (plus.x - minus.x));
         }
         const int64_t ia = alm_index(l, m, lmax);
-        if (mode) {
+        if (mode == 1) {
             ae[ia] = csub(ae[ia], E);
             ab[ia] = csub(ab[ia], B);
+        } else if (mode == 2) {
+            ae[ia] = cadd(ae[ia], E);
+            ab[ia] = cadd(ab[ia], B);
         } else {
             ae[ia] = E;
             ab[ia] = B;
@@ -938,13 +953,14 @@ LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn, int K =
     LatConfig cfg;
     // the launchers select the instantiation by NW: NW 1 -> R 2, NW 2 -> R 4, NW 4 -> the
     // large R; CH = 64 NW R must match the kernel's own chunk size
-    if (nrings <= 128) cfg = {2, 1, 128, 0};
-    else if (nrings <= 512) cfg = {4, 2, 512, 0};
+    if (nrings <= 128) cfg = {2, 1, 128, 0, 0};
+    else if (nrings <= 512) cfg = {4, 2, 512, 0, 0};
     else {
         const int r = large_r(syn, mixed, spin == 0, K);
-        cfg = {r, 4, 256 * r, 0};
+        cfg = {r, 4, 256 * r, 0, 0};
     }
     cfg.nchunks = (int)((nrings + cfg.CH - 1) / cfg.CH);
+    cfg.nslots = (nside <= kOnePassNside || cfg.nchunks <= kSlotsPerPass) ? cfg.nchunks : kSlotsPerPass;
     return cfg;
 }
 
@@ -981,10 +997,14 @@ using namespace gm;
 
 extern "C" int64_t gm_table_size(int64_t lmax) { return table_size(lmax + 1); }
 
-extern "C" int64_t gm_lat_nchunks(int64_t nside, int64_t spin) { return lat_config(nside, spin).nchunks; }
+extern "C" int64_t gm_lat_nchunks(int64_t nside, int64_t spin) { return lat_config(nside, spin).nslots; }
 
 extern "C" int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin, int64_t nmaps) {
-    return lat_config(nside, spin, true, (int)nmaps).nchunks;
+    return lat_config(nside, spin, true, (int)nmaps).nslots;
+}
+
+extern "C" int64_t gm_lat_ring_chunks(int64_t nside, int64_t spin, int64_t mixed, int64_t nmaps) {
+    return lat_config(nside, spin, mixed != 0, (int)nmaps).nchunks;
 }
 
 extern "C" int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* gamma, double* P, void* stream) {
@@ -1003,7 +1023,7 @@ namespace {
 template <typename F>
 int launch_ana_s0(const char* name, const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
                   int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
-                  const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+                  const int32_t* nactive, double* partial, int64_t nchunks, int64_t chunk0, void* stream) {
     if (int e = check_lat(nside, lmax, 0)) return e;
     if (!spec || !alpha || !seed_mant || !seed_exp || !nactive || !partial) return fail(GM_E_ARG, "%s: null pointer", name);
     constexpr bool mixed = std::is_same<F, float>::value;
@@ -1013,15 +1033,18 @@ int launch_ana_s0(const char* name, const double* spec, int64_t nmaps, int64_t l
     if (ld < L) return fail(GM_E_SHAPE, "%s: ld < lmax + 1", name);
     if (nmaps > 1 && smap < (4 * nside - 1) * ld) return fail(GM_E_SHAPE, "%s: map stride too small", name);
     const LatConfig cfg = lat_config(nside, 0, mixed, (int)nmaps);
-    if (nchunks != cfg.nchunks)
-        return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
+    if (nchunks != cfg.nslots)
+        return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nslots);
+    if (chunk0 < 0 || chunk0 >= cfg.nchunks || chunk0 % cfg.nslots)
+        return fail(GM_E_ARG, "%s: chunk0 %lld is not a pass start below %d", name, (long long)chunk0, cfg.nchunks);
     const int64_t ts = table_size(L);
-    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    const int nblk = cfg.nchunks - (int)chunk0 < cfg.nslots ? cfg.nchunks - (int)chunk0 : cfg.nslots;
+    dim3 grid(nblk, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* sp = reinterpret_cast<const double2*>(spec);
     constexpr int RL = large_r(false, mixed, true);
     if (int e = check_config(name, cfg, large_r(false, mixed, true, (int)nmaps))) return e;
-#define GM_ANA0(KK, RR, WW) hipLaunchKernelGGL((k_ana_s0<KK, RR, WW, F>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts)
+#define GM_ANA0(KK, RR, WW) hipLaunchKernelGGL((k_ana_s0<KK, RR, WW, F>), grid, block, 0, st, sp, ld, smap, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, partial, ts, (int)chunk0)
     if (nmaps == 1) {
         if (cfg.NW == 1) GM_ANA0(1, 2, 1);
         else if (cfg.NW == 2) GM_ANA0(1, 4, 2);
@@ -1099,7 +1122,7 @@ int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t n
 template <typename F>
 int launch_ana_s(const char* name, const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
                  const double* alpha, const double* gamma, const double* seed_mant, const int32_t* seed_exp,
-                 const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+                 const int32_t* nactive, double* partial, int64_t nchunks, int64_t chunk0, void* stream) {
     if (int e = check_lat(nside, lmax, spin)) return e;
     if (spin == 0) return fail(GM_E_ARG, "%s: use the spin-0 entry for spin 0", name);
     if (!spec || !alpha || !gamma || !seed_mant || !seed_exp || !nactive || !partial)
@@ -1107,15 +1130,18 @@ int launch_ana_s(const char* name, const double* spec, int64_t ld, int64_t nside
     const int64_t L = lmax + 1;
     if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "%s: ld < 2 lmax + 1", name);
     const LatConfig cfg = lat_config(nside, spin, std::is_same<F, float>::value);
-    if (nchunks != cfg.nchunks)
-        return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nchunks);
-    dim3 grid(cfg.nchunks, (unsigned)L), block(64 * cfg.NW);
+    if (nchunks != cfg.nslots)
+        return fail(GM_E_SHAPE, "%s: partial has %lld chunks, need %d", name, (long long)nchunks, cfg.nslots);
+    if (chunk0 < 0 || chunk0 >= cfg.nchunks || chunk0 % cfg.nslots)
+        return fail(GM_E_ARG, "%s: chunk0 %lld is not a pass start below %d", name, (long long)chunk0, cfg.nchunks);
+    const int nblk = cfg.nchunks - (int)chunk0 < cfg.nslots ? cfg.nchunks - (int)chunk0 : cfg.nslots;
+    dim3 grid(nblk, (unsigned)L), block(64 * cfg.NW);
     hipStream_t st = (hipStream_t)stream;
     const double2* sp = reinterpret_cast<const double2*>(spec);
     const int64_t ts = table_size(L);
     constexpr int RL = large_r(false, std::is_same<F, float>::value, false);
     if (int e = check_config(name, cfg, RL)) return e;
-#define GM_ANAS(RR, WW) hipLaunchKernelGGL((k_ana_s<RR, WW, F>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts)
+#define GM_ANAS(RR, WW) hipLaunchKernelGGL((k_ana_s<RR, WW, F>), grid, block, 0, st, sp, ld, (int)nside, (int)L, (int)spin, alpha, gamma, seed_mant, seed_exp, nactive, partial, ts, (int)chunk0)
     if (cfg.NW == 1) GM_ANAS(2, 1);
     else if (cfg.NW == 2) GM_ANAS(4, 2);
     else GM_ANAS(RL, 4);
@@ -1161,23 +1187,26 @@ int launch_syn_s(const char* name, const double* coef, int64_t nside, int64_t lm
 
 extern "C" int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
                              int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
-                             const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+                             const int32_t* nactive, double* partial, int64_t nchunks, int64_t chunk0,
+                             void* stream) {
     return launch_ana_s0<double>("gm_lat_ana_s0", spec, nmaps, ld, smap, nside, lmax, alpha, seed_mant, seed_exp,
-                                 nactive, partial, nchunks, stream);
+                                 nactive, partial, nchunks, chunk0, stream);
 }
 
 extern "C" int gm_lat_ana_s0_mixed(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside,
                                    int64_t lmax, const double* alpha, const double* seed_mant, const int32_t* seed_exp,
-                                   const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+                                   const int32_t* nactive, double* partial, int64_t nchunks, int64_t chunk0,
+                                   void* stream) {
     return launch_ana_s0<float>("gm_lat_ana_s0_mixed", spec, nmaps, ld, smap, nside, lmax, alpha, seed_mant, seed_exp,
-                                nactive, partial, nchunks, stream);
+                                nactive, partial, nchunks, chunk0, stream);
 }
 
 namespace gm {
 namespace {
 int finish_s0(const char* name, bool mixed, const double* partial, int64_t nmaps, int64_t nside,
               const int32_t* nactive, const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
-              void* stream) {
+              int64_t chunk0, void* stream) {
+    if (mode < 0 || mode > 2 || chunk0 < 0) return fail(GM_E_ARG, "%s: bad mode / chunk0", name);
     if (int e = check_lat(nside, lmax, 0)) return e;
     if (!partial || !nactive || !P || !alm) return fail(GM_E_ARG, "%s: null pointer", name);
     if (!(nmaps == 1 || nmaps == 2 || (mixed && (nmaps == 4 || nmaps == 8))))
@@ -1186,8 +1215,8 @@ int finish_s0(const char* name, bool mixed, const double* partial, int64_t nmaps
     if (nmaps > 1 && alm_map_stride < nelem) return fail(GM_E_SHAPE, "%s: alm stride too small", name);
     const LatConfig cfg = lat_config(nside, 0, mixed, (int)nmaps);
     hipLaunchKernelGGL(k_finish_s0, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const double2*>(partial), (int)nmaps, cfg.nchunks, cfg.CH, nactive, P, (int)L,
-                       table_size(L), reinterpret_cast<double2*>(alm), alm_map_stride, mode);
+                       reinterpret_cast<const double2*>(partial), (int)nmaps, cfg.nslots, cfg.CH, nactive, P, (int)L,
+                       table_size(L), reinterpret_cast<double2*>(alm), alm_map_stride, mode, (int)chunk0);
     return check_launch(name);
 }
 }  // namespace
@@ -1195,16 +1224,16 @@ int finish_s0(const char* name, bool mixed, const double* partial, int64_t nmaps
 
 extern "C" int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
                                 const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
-                                void* stream) {
+                                int64_t chunk0, void* stream) {
     return finish_s0("gm_lat_finish_s0", false, partial, nmaps, nside, nactive, P, lmax, alm, alm_map_stride, mode,
-                     stream);
+                     chunk0, stream);
 }
 
 extern "C" int gm_lat_finish_s0_mixed(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
                                       const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
-                                      void* stream) {
+                                      int64_t chunk0, void* stream) {
     return finish_s0("gm_lat_finish_s0_mixed", true, partial, nmaps, nside, nactive, P, lmax, alm, alm_map_stride,
-                     mode, stream);
+                     mode, chunk0, stream);
 }
 
 extern "C" int gm_lat_prep_s0(const double* alm, int64_t nmaps, int64_t alm_map_stride, const double* P,
@@ -1236,44 +1265,51 @@ extern "C" int gm_lat_syn_s0_mixed(const double* coef, int64_t nmaps, int64_t ns
 
 extern "C" int gm_lat_ana_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
                             const double* alpha, const double* gamma, const double* seed_mant, const int32_t* seed_exp,
-                            const int32_t* nactive, double* partial, int64_t nchunks, void* stream) {
+                            const int32_t* nactive, double* partial, int64_t nchunks, int64_t chunk0,
+                            void* stream) {
     return launch_ana_s<double>("gm_lat_ana_s", spec, ld, nside, lmax, spin, alpha, gamma, seed_mant, seed_exp,
-                                nactive, partial, nchunks, stream);
+                                nactive, partial, nchunks, chunk0, stream);
 }
 
 extern "C" int gm_lat_ana_s_mixed(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
                                   const double* alpha, const double* gamma, const double* seed_mant,
                                   const int32_t* seed_exp, const int32_t* nactive, double* partial, int64_t nchunks,
-                                  void* stream) {
+                                  int64_t chunk0, void* stream) {
     return launch_ana_s<float>("gm_lat_ana_s_mixed", spec, ld, nside, lmax, spin, alpha, gamma, seed_mant, seed_exp,
-                               nactive, partial, nchunks, stream);
+                               nactive, partial, nchunks, chunk0, stream);
 }
 
 namespace gm {
 namespace {
 int finish_s(const char* name, bool mixed, const double* partial, int64_t nside, const int32_t* nactive,
-             const double* P, int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream) {
+             const double* P, int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, int64_t chunk0,
+             void* stream) {
+    if (mode < 0 || mode > 2 || chunk0 < 0) return fail(GM_E_ARG, "%s: bad mode / chunk0", name);
     if (int e = check_lat(nside, lmax, spin)) return e;
     if (!partial || !nactive || !P || !alm_e || !alm_b) return fail(GM_E_ARG, "%s: null pointer", name);
     const int64_t L = lmax + 1;
     const LatConfig cfg = lat_config(nside, spin, mixed);
     hipLaunchKernelGGL(k_finish_s, dim3((unsigned)((L + 255) / 256), (unsigned)L), dim3(256), 0, (hipStream_t)stream,
-                       reinterpret_cast<const double2*>(partial), cfg.nchunks, cfg.CH, nactive, P, (int)L, (int)spin,
-                       table_size(L), reinterpret_cast<double2*>(alm_e), reinterpret_cast<double2*>(alm_b), mode);
+                       reinterpret_cast<const double2*>(partial), cfg.nslots, cfg.CH, nactive, P, (int)L, (int)spin,
+                       table_size(L), reinterpret_cast<double2*>(alm_e), reinterpret_cast<double2*>(alm_b), mode,
+                       (int)chunk0);
     return check_launch(name);
 }
 }  // namespace
 }  // namespace gm
 
 extern "C" int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
-                               int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream) {
-    return finish_s("gm_lat_finish_s", false, partial, nside, nactive, P, lmax, spin, alm_e, alm_b, mode, stream);
+                               int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, int64_t chunk0,
+                               void* stream) {
+    return finish_s("gm_lat_finish_s", false, partial, nside, nactive, P, lmax, spin, alm_e, alm_b, mode, chunk0,
+                    stream);
 }
 
 extern "C" int gm_lat_finish_s_mixed(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
                                      int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode,
-                                     void* stream) {
-    return finish_s("gm_lat_finish_s_mixed", true, partial, nside, nactive, P, lmax, spin, alm_e, alm_b, mode, stream);
+                                     int64_t chunk0, void* stream) {
+    return finish_s("gm_lat_finish_s_mixed", true, partial, nside, nactive, P, lmax, spin, alm_e, alm_b, mode, chunk0,
+                    stream);
 }
 
 extern "C" int gm_lat_prep_s(const double* alm_e, const double* alm_b, const double* P, int64_t lmax, int64_t spin,

@@ -374,6 +374,235 @@ __global__ void k_ring_inv_s(const double2* __restrict__ spec, int64_t ld, int n
     }
 }
 
+// ----------------------------------------------------------------------- rings above LDS
+// A ring whose packed length N = nphi / 2 exceeds kLdsN (Nside 8192: the belt has N =
+// 16384, caps up to 16382) is transformed in two halves of H = N / 2 points, one parity of
+// the output index at a time (one radix-2 decimation-in-frequency step outside LDS):
+//   Z[2k]   = FFT_H(z[j] + z[j+H])[k],   Z[2k+1] = FFT_H((z[j] - z[j+H]) e^{-2 pi i j/N})[k].
+// The packed-real post-processing pairs K with N - K, which have the same parity, and the
+// phased output m reads X[m mod nphi], whose parity is that of m, so each parity pass
+// finishes its own outputs.  The inverse runs the same split backwards: y_p = IFFT_H of
+// the parity-p half, then z[j] = y_0 + w^j y_1, z[j+H] = y_0 - w^j y_1 (w = e^{2 pi i/N}),
+// the second pass updating in place what the same thread wrote in the first.
+#ifndef GM_RING_LDS_N
+#define GM_RING_LDS_N 8192
+#endif
+constexpr int kLdsN = GM_RING_LDS_N;  // complex FP64 points per workgroup in LDS (128 KiB)
+
+// buf[k] = Z[2k+p] (k < H) -> X[2k+p] in place; for p = 0 also X[N] through *xn.
+__device__ void r2c_post_half(double2* buf, int H, int p, double2* xn) {
+    const int N = 2 * H, n = 2 * N;
+    for (int k = threadIdx.x; k < H; k += blockDim.x) {
+        const int kp = p ? H - 1 - k : (H - k) % H;  // slot of Z[N - K]
+        if (kp < k) continue;
+        if (p == 0 && k == 0) {
+            const double2 z = buf[0];
+            buf[0] = make_double2(z.x + z.y, 0.0);
+            *xn = make_double2(z.x - z.y, 0.0);
+            continue;
+        }
+        const int K = 2 * k + p, KK = 2 * kp + p;
+        const double2 a = buf[k], b = buf[kp];
+        const double2 e = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+        const double2 o = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y + b.y));
+        const double2 wo = cmul(twiddle(K, n, -1.0), o);
+        const double2 xk = make_double2(e.x + wo.y, e.y - wo.x);
+        if (kp != k) {
+            const double2 e2 = make_double2(e.x, -e.y);
+            const double2 o2 = make_double2(-o.x, o.y);
+            const double2 wo2 = cmul(twiddle(KK, n, -1.0), o2);
+            buf[kp] = make_double2(e2.x + wo2.y, e2.y - wo2.x);
+        }
+        buf[k] = xk;
+    }
+    __syncthreads();
+}
+
+// buf[k] = X[2k+p] (k < H; X[N] in xn) -> Z[2k+p] in place.
+__device__ void c2r_pre_half(double2* buf, int H, int p, double2 xn) {
+    const int N = 2 * H, n = 2 * N;
+    for (int k = threadIdx.x; k < H; k += blockDim.x) {
+        const int kp = p ? H - 1 - k : (H - k) % H;
+        if (kp < k) continue;
+        const int K = 2 * k + p, KK = 2 * kp + p;
+        const double2 a = buf[k];
+        const double2 b = (p == 0 && k == 0) ? xn : buf[kp];
+        const double2 s = make_double2(a.x + b.x, a.y - b.y);
+        const double2 d = make_double2(a.x - b.x, a.y + b.y);
+        const double2 wd = cmul(twiddle(K, n, 1.0), d);
+        const double2 zk = make_double2(s.x - wd.y, s.y + wd.x);
+        if (!(p == 0 && k == 0) && kp != k) {
+            const double2 s2 = make_double2(s.x, -s.y);
+            const double2 d2 = make_double2(-d.x, d.y);
+            const double2 wd2 = cmul(twiddle(KK, n, 1.0), d2);
+            buf[kp] = make_double2(s2.x - wd2.y, s2.y + wd2.x);
+        }
+        buf[k] = zk;
+    }
+    __syncthreads();
+}
+
+// Half of the packed complex sequence of a real ring x (z[j] = x[2j] + i x[2j+1]) for
+// parity pass p, into buf[0..H).
+__device__ inline void load_half(double2* buf, const double* x, int H, int p) {
+    const int N = 2 * H;
+    for (int k = threadIdx.x; k < H; k += blockDim.x) {
+        const double2 a = make_double2(x[2 * k], x[2 * k + 1]);
+        const double2 b = make_double2(x[2 * (k + H)], x[2 * (k + H) + 1]);
+        buf[k] = p ? cmul(csub(a, b), twiddle(k, N, -1.0)) : cadd(a, b);
+    }
+    __syncthreads();
+}
+
+// X[kk] (parity p) from the parity-p half spectrum in buf (X[N] in xn)
+__device__ inline double2 half_parity_x(const double2* buf, double2 xn, int N, int p, int kk) {
+    if (kk < N) return buf[(kk - p) >> 1];
+    if (kk == N) return xn;
+    return cconj(buf[(2 * N - kk - p) >> 1]);
+}
+
+__global__ void k_ring_fwd_s0_split(const double* __restrict__ maps, int64_t map_stride, int nside, int L, int qlo,
+                                    int qhi, double2* __restrict__ spec, int64_t ld, int64_t spec_map_stride) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2, H = N / 2;
+    const double* x = maps + blockIdx.y * map_stride + g.offset;
+    const double w = 4.0 * M_PI / (12.0 * (double)nside * (double)nside);
+    double2* out = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
+    for (int p = 0; p < 2; ++p) {
+        load_half(buf, x, H, p);
+        fft_lds(buf, H, -1.0);
+        r2c_post_half(buf, H, p, &xn);
+        for (int m = p + 2 * threadIdx.x; m < L; m += 2 * blockDim.x) {
+            const double2 X = half_parity_x(buf, xn, N, p, m % n);
+            out[m] = cscale(cmul(phase_pi(m, g.phi0_num, g.phi0_den, -1.0), X), w);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_ring_fwd_s_split(const double* __restrict__ q, const double* __restrict__ u, int nside, int L,
+                                   int qlo, int qhi, double2* __restrict__ spec, int64_t ld) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2, H = N / 2;
+    const double w = 4.0 * M_PI / (12.0 * (double)nside * (double)nside);
+    double2* out = spec + (int64_t)r * ld;
+    const int W = 2 * L - 1;
+    for (int comp = 0; comp < 2; ++comp) {
+        const double* x = (comp == 0 ? q : u) + g.offset;
+        for (int p = 0; p < 2; ++p) {
+            load_half(buf, x, H, p);
+            fft_lds(buf, H, -1.0);
+            r2c_post_half(buf, H, p, &xn);
+            // outputs m = j - (L - 1) of parity p; the same thread owns j in both components
+            for (int j = ((L - 1 + p) & 1) + 2 * threadIdx.x; j < W; j += 2 * blockDim.x) {
+                const int m = j - (L - 1);
+                int kk = m % n;
+                if (kk < 0) kk += n;
+                const double2 X = half_parity_x(buf, xn, N, p, kk);
+                const double2 v = cscale(cmul(phase_pi(m, g.phi0_num, g.phi0_den, -1.0), X), w);
+                if (comp == 0) {
+                    out[j] = v;
+                } else {
+                    const double2 o = out[j];
+                    out[j] = make_double2(o.x - v.y, o.y + v.x);  // + i v
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// z[j] and z[j+H] of the packed ring from the parity-p inverse half y_p in buf: pass 0
+// stores y_0 in both, pass 1 adds +-w^j y_1 to what this thread stored in pass 0.
+__device__ inline void store_half(double* x, const double2* buf, int H, int p) {
+    const int N = 2 * H;
+    for (int k = threadIdx.x; k < H; k += blockDim.x) {
+        const double2 y = buf[k];
+        if (p == 0) {
+            x[2 * k] = y.x;
+            x[2 * k + 1] = y.y;
+            x[2 * (k + H)] = y.x;
+            x[2 * (k + H) + 1] = y.y;
+        } else {
+            const double2 t = cmul(twiddle(k, N, 1.0), y);
+            const double2 y0 = make_double2(x[2 * k], x[2 * k + 1]);
+            x[2 * k] = y0.x + t.x;
+            x[2 * k + 1] = y0.y + t.y;
+            x[2 * (k + H)] = y0.x - t.x;
+            x[2 * (k + H) + 1] = y0.y - t.y;
+        }
+    }
+}
+
+__global__ void k_ring_inv_s0_split(const double2* __restrict__ spec, int64_t ld, int64_t spec_map_stride, int nside,
+                                    int L, int qlo, int qhi, double* __restrict__ maps, int64_t map_stride) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2, H = N / 2;
+    const double2* F = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
+    double* x = maps + blockIdx.y * map_stride + g.offset;
+    for (int p = 0; p < 2; ++p) {
+        for (int kh = threadIdx.x; kh <= H; kh += blockDim.x) {
+            if (kh == H && p == 1) continue;
+            const int k = kh == H ? N : 2 * kh + p;  // kh == H carries X[N] (parity 0)
+            const double2 a = fold_sum(F, 0, 0, L - 1, k, n, g);
+            const double2 b = fold_sum(F, 0, 1, L - 1, (n - k) % n, n, g);
+            const double2 X = make_double2(a.x + b.x, a.y - b.y);
+            if (kh < H) buf[kh] = X;
+            else xn = X;
+        }
+        __syncthreads();
+        c2r_pre_half(buf, H, p, xn);
+        fft_lds(buf, H, 1.0);
+        store_half(x, buf, H, p);
+        __syncthreads();
+    }
+}
+
+__global__ void k_ring_inv_s_split(const double2* __restrict__ spec, int64_t ld, int nside, int L, int qlo, int qhi,
+                                   double* __restrict__ q, double* __restrict__ u) {
+    extern __shared__ double2 buf[];
+    __shared__ double2 xn;
+    const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
+    const Ring g = ring_info(nside, r);
+    const int n = g.nphi, N = n / 2, H = N / 2;
+    const double2* F = spec + (int64_t)r * ld;
+    const int64_t col0 = L - 1;
+    for (int comp = 0; comp < 2; ++comp) {
+        double* x = (comp == 0 ? q : u) + g.offset;
+        for (int p = 0; p < 2; ++p) {
+            for (int kh = threadIdx.x; kh <= H; kh += blockDim.x) {
+                if (kh == H && p == 1) continue;
+                const int k = kh == H ? N : 2 * kh + p;
+                const double2 yp = fold_sum(F, col0, -(L - 1), L - 1, k, n, g);
+                const double2 ym = fold_sum(F, col0, -(L - 1), L - 1, (n - k) % n, n, g);
+                double2 X;
+                if (comp == 0) {
+                    X = make_double2(0.5 * (yp.x + ym.x), 0.5 * (yp.y - ym.y));
+                } else {
+                    const double2 d = make_double2(yp.x - ym.x, yp.y + ym.y);
+                    X = make_double2(0.5 * d.y, -0.5 * d.x);
+                }
+                if (kh < H) buf[kh] = X;
+                else xn = X;
+            }
+            __syncthreads();
+            c2r_pre_half(buf, H, p, xn);
+            fft_lds(buf, H, 1.0);
+            store_half(x, buf, H, p);
+            __syncthreads();
+        }
+    }
+}
+
 // ----------------------------------------------------------------------- fold residual
 // One workgroup per ring, one thread per residue class c of m modulo nphi.
 __global__ void k_fold_s0(const double2* __restrict__ F, const double2* G, double2* R, int64_t ld,
@@ -490,7 +719,7 @@ inline int threads_for(int N) {
     return t;
 }
 
-constexpr int kMaxNside = 4096;
+constexpr int kMaxNside = 8192;
 
 template <typename K>
 int set_lds(K kernel, size_t bytes) {
@@ -528,9 +757,11 @@ extern "C" int gm_ring_fft_fwd_s0(const double* maps, int64_t nmaps, int64_t map
     Bucket b[32];
     const int nb = make_buckets((int)nside, b);
     for (int i = 0; i < nb; ++i) {
-        const size_t lds = (size_t)b[i].N * sizeof(double2);
-        if (int e = set_lds(k_ring_fwd_s0, lds)) return e;
-        hipLaunchKernelGGL(k_ring_fwd_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
+        const bool split = b[i].N > kLdsN;
+        const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
+        const size_t lds = (size_t)NL * sizeof(double2);
+        if (int e = set_lds(split ? k_ring_fwd_s0_split : k_ring_fwd_s0, lds)) return e;
+        hipLaunchKernelGGL(split ? k_ring_fwd_s0_split : k_ring_fwd_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
                            (hipStream_t)stream, maps, map_stride, (int)nside, (int)L, b[i].qlo, b[i].qhi,
                            reinterpret_cast<double2*>(spec), ld, spec_map_stride);
     }
@@ -546,9 +777,11 @@ extern "C" int gm_ring_fft_fwd_s(const double* q, const double* u, int64_t nside
     Bucket b[32];
     const int nb = make_buckets((int)nside, b);
     for (int i = 0; i < nb; ++i) {
-        const size_t lds = (size_t)b[i].N * sizeof(double2);
-        if (int e = set_lds(k_ring_fwd_s, lds)) return e;
-        hipLaunchKernelGGL(k_ring_fwd_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
+        const bool split = b[i].N > kLdsN;
+        const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
+        const size_t lds = (size_t)NL * sizeof(double2);
+        if (int e = set_lds(split ? k_ring_fwd_s_split : k_ring_fwd_s, lds)) return e;
+        hipLaunchKernelGGL(split ? k_ring_fwd_s_split : k_ring_fwd_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
                            q, u, (int)nside, (int)L, b[i].qlo, b[i].qhi, reinterpret_cast<double2*>(spec), ld);
     }
     return check_launch("gm_ring_fft_fwd_s");
@@ -566,9 +799,11 @@ extern "C" int gm_ring_fft_inv_s0(const double* spec, int64_t nmaps, int64_t ld,
     Bucket b[32];
     const int nb = make_buckets((int)nside, b);
     for (int i = 0; i < nb; ++i) {
-        const size_t lds = (size_t)b[i].N * sizeof(double2);
-        if (int e = set_lds(k_ring_inv_s0, lds)) return e;
-        hipLaunchKernelGGL(k_ring_inv_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
+        const bool split = b[i].N > kLdsN;
+        const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
+        const size_t lds = (size_t)NL * sizeof(double2);
+        if (int e = set_lds(split ? k_ring_inv_s0_split : k_ring_inv_s0, lds)) return e;
+        hipLaunchKernelGGL(split ? k_ring_inv_s0_split : k_ring_inv_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
                            (hipStream_t)stream, reinterpret_cast<const double2*>(spec), ld, spec_map_stride,
                            (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride);
     }
@@ -584,9 +819,11 @@ extern "C" int gm_ring_fft_inv_s(const double* spec, int64_t ld, int64_t nside, 
     Bucket b[32];
     const int nb = make_buckets((int)nside, b);
     for (int i = 0; i < nb; ++i) {
-        const size_t lds = (size_t)b[i].N * sizeof(double2);
-        if (int e = set_lds(k_ring_inv_s, lds)) return e;
-        hipLaunchKernelGGL(k_ring_inv_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
+        const bool split = b[i].N > kLdsN;
+        const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
+        const size_t lds = (size_t)NL * sizeof(double2);
+        if (int e = set_lds(split ? k_ring_inv_s_split : k_ring_inv_s, lds)) return e;
+        hipLaunchKernelGGL(split ? k_ring_inv_s_split : k_ring_inv_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
                            reinterpret_cast<const double2*>(spec), ld, (int)nside, (int)L, b[i].qlo, b[i].qhi,
                            q, u);
     }

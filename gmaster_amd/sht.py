@@ -144,12 +144,28 @@ def _check_nside(npix):
     return nside
 
 
+def _make_room(nside):
+    """Above Nside 4096 one transform's temporaries reach ~100 GB (three ring spectra and
+    the partial sums of a spin-2 field): hand cached but unused blocks back first, so the
+    allocator is not left fragmented by the previous transform's differently sized ones."""
+    if nside > 4096:
+        torch.cuda.empty_cache()
+
+
 def _as_f64(t, device):
     t = torch.as_tensor(t)
     return t.to(device=device, dtype=torch.float64).contiguous()
 
 
 # ------------------------------------------------------------------------------ spin 0
+def _passes(p, spin, mixed, K, nslots, mode):
+    """``(chunk0, finish mode)`` of each analysis pass: one pass up to Nside 4096; above, the
+    ring chunks are covered ``nslots`` at a time so the partial buffer stays bounded, the
+    later passes adding to (or, for the Jacobi update, subtracting from) the alms."""
+    total = int(_lib.query("gm_lat_ring_chunks", p.nside, spin, int(bool(mixed)), K))
+    return [(c0, mode if c0 == 0 else (1 if mode == 1 else 2)) for c0 in range(0, total, nslots)]
+
+
 def _analysis_s0(p, spec, alm, mode, mixed=False):
     K = spec.shape[0]
     st = _lib.stream_ptr(p.device)
@@ -157,10 +173,11 @@ def _analysis_s0(p, spec, alm, mode, mixed=False):
     sfx = "_mixed" if mixed else ""
     partial = torch.empty(nch * p.tsize * 2 * K, dtype=torch.float64, device=p.device)
     ld = spec.shape[2]
-    _lib.call("gm_lat_ana_s0" + sfx, _lib.ptr(spec), K, ld, spec.stride(0), p.nside, p.lmax, _lib.ptr(p.alpha),
-              _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), nch, st)
-    _lib.call("gm_lat_finish_s0" + sfx, _lib.ptr(partial), K, p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax,
-              _lib.ptr(alm), alm.stride(0), int(mode), st)
+    for c0, md in _passes(p, 0, mixed, K, nch, int(mode)):
+        _lib.call("gm_lat_ana_s0" + sfx, _lib.ptr(spec), K, ld, spec.stride(0), p.nside, p.lmax, _lib.ptr(p.alpha),
+                  _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), nch, c0, st)
+        _lib.call("gm_lat_finish_s0" + sfx, _lib.ptr(partial), K, p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax,
+                  _lib.ptr(alm), alm.stride(0), md, c0, st)
 
 
 def _synthesis_s0(p, alm, spec, mixed=False):
@@ -192,9 +209,10 @@ def _refine(alm, n_iter, analysis, synthesis, fold, spec, mixed):
             fold(synth, spec, resid)
             analysis(resid, alm, 1, False)
         return alm
-    ra, rb = torch.empty_like(spec), torch.empty_like(spec)
+    ra = torch.empty_like(spec)
     synthesis(alm, synth, True)
     fold(synth, spec, ra)
+    rb = spec  # the input spectrum is not read again: its buffer holds the next residual
     sign = 1
     d = torch.empty_like(alm)
     for it in range(int(n_iter)):
@@ -240,6 +258,7 @@ def map2alm_s0(maps, lmax, n_iter=0, polar_skip=True, precision="fp64", max_grou
     K, npix = maps.shape
     nside = _check_nside(npix)
     lmax = int(lmax)
+    _make_room(nside)
     p = plan(nside, lmax, 0, dev, polar_skip)
     L = lmax + 1
     nring = 4 * nside - 1
@@ -294,10 +313,12 @@ def _analysis_s(p, spec, ae, ab, mode, mixed=False):
     nch = p.nchunks_mixed if mixed else p.nchunks
     sfx = "_mixed" if mixed else ""
     partial = torch.empty(nch * p.tsize * 4, dtype=torch.float64, device=p.device)
-    _lib.call("gm_lat_ana_s" + sfx, _lib.ptr(spec), spec.shape[1], p.nside, p.lmax, p.spin, _lib.ptr(p.alpha),
-              _lib.ptr(p.gamma), _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), nch, st)
-    _lib.call("gm_lat_finish_s" + sfx, _lib.ptr(partial), p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax, p.spin,
-              _lib.ptr(ae), _lib.ptr(ab), int(mode), st)
+    for c0, md in _passes(p, p.spin, mixed, 1, nch, int(mode)):
+        _lib.call("gm_lat_ana_s" + sfx, _lib.ptr(spec), spec.shape[1], p.nside, p.lmax, p.spin, _lib.ptr(p.alpha),
+                  _lib.ptr(p.gamma), _lib.ptr(p.smant), _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(partial), nch,
+                  c0, st)
+        _lib.call("gm_lat_finish_s" + sfx, _lib.ptr(partial), p.nside, _lib.ptr(p.nact), _lib.ptr(p.P), p.lmax,
+                  p.spin, _lib.ptr(ae), _lib.ptr(ab), md, c0, st)
 
 
 def _synthesis_s(p, ae, ab, spec, mixed=False):
@@ -319,6 +340,7 @@ def map2alm_spin(maps, lmax, spin, n_iter=0, polar_skip=True, precision="fp64"):
     npix = maps.shape[1]
     nside = _check_nside(npix)
     lmax, spin = int(lmax), int(spin)
+    _make_room(nside)
     p = plan(nside, lmax, spin, dev, polar_skip)
     L = lmax + 1
     W = 2 * L - 1

@@ -91,19 +91,23 @@ int gm_ring_fold_residual_c(const double* F, const double* G, double* R, int64_t
  * Replaces the window tables of gmaster/_sht/march_v2.py:388-450. */
 int gm_lat_tables(int64_t lmax, int64_t spin, double* alpha, double* gamma, double* P, void* stream);
 
-/* Spin-0 analysis of nmaps (1 or 2) phased spectra sharing one recurrence:
- * partial[c][base(m) + l - m][map] = sum_{rings of chunk c} mu_lm(theta) (G_N +- G_S)
+/* Spin-0 analysis of nmaps (1 or 2) phased spectra sharing one recurrence, one pass over
+ * the ring chunks chunk0 .. chunk0 + nchunks - 1:
+ * partial[c - chunk0][base(m) + l - m][map] = sum_{rings of chunk c} mu_lm(theta) (G_N +- G_S)
+ * nchunks = gm_lat_nchunks(nside, 0) slots; chunk0 = 0, nchunks, 2 nchunks, ... below
+ * gm_lat_ring_chunks (one pass up to Nside 4096; passes bound the partial buffer above).
  * Replaces gm_march_ana_s0 / gm_march_ana_s0_pair (march_v2.cu:155-285, :551-552). */
 int gm_lat_ana_s0(const double* spec, int64_t nmaps, int64_t ld, int64_t spec_map_stride,
                   int64_t nside, int64_t lmax, const double* alpha, const double* seed_mant,
                   const int32_t* seed_exp, const int32_t* nactive, double* partial,
-                  int64_t nchunks, void* stream);
+                  int64_t nchunks, int64_t chunk0, void* stream);
 
-/* Sum the ring-chunk partials of the analysis (chunking is a function of nside), scale by
- * P and write alm (mode 0: alm = result; 1: alm -= result, the Jacobi update). */
+/* Sum the partial slots of the analysis pass that started at chunk0, scale by P and write
+ * alm (mode 0: alm = result; 1: alm -= result, the Jacobi update; 2: alm += result, the
+ * later passes of a multi-pass analysis). */
 int gm_lat_finish_s0(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
                      const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
-                     void* stream);
+                     int64_t chunk0, void* stream);
 
 /* Synthesis coefficients: coef[base(m)+l-m][map] = alm[map][idx(l,m)] P_lm (zero padded);
  * nmaps 1, 2, 4 or 8 (4 and 8 feed the batched mixed synthesis). */
@@ -116,17 +120,20 @@ int gm_lat_syn_s0(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax
                   const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
                   double* spec, int64_t ld, int64_t spec_map_stride, void* stream);
 
-/* Spin-s analysis over all rings (direct + mirror channel per order m >= 0).
+/* Spin-s analysis over all rings (direct + mirror channel per order m >= 0), one pass over
+ * ring chunks chunk0 .. chunk0 + nchunks - 1 as for spin 0.
  * Replaces gm_march_ana_s2 (march_v2.cu:553). */
 int gm_lat_ana_s(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin,
                  const double* alpha, const double* gamma, const double* seed_mant,
                  const int32_t* seed_exp, const int32_t* nactive, double* partial, int64_t nchunks,
-                 void* stream);
+                 int64_t chunk0, void* stream);
 
-/* Finish spin s: sum partials, scale, combine +-m into E/B (healpy packing, 2 x nelem).
+/* Finish spin s: sum the pass's partials, scale, combine +-m into E/B (healpy packing,
+ * 2 x nelem); modes as gm_lat_finish_s0.
  * Replaces gmaster/_sht/healpix.py:117-143 and :812-814. */
 int gm_lat_finish_s(const double* partial, int64_t nside, const int32_t* nactive, const double* P,
-                    int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, void* stream);
+                    int64_t lmax, int64_t spin, double* alm_e, double* alm_b, int mode, int64_t chunk0,
+                    void* stream);
 
 /* Synthesis coefficients for spin s from E/B (gmaster/_sht/healpix.py:599-611). */
 int gm_lat_prep_s(const double* alm_e, const double* alm_b, const double* P, int64_t lmax,
@@ -137,8 +144,11 @@ int gm_lat_syn_s(const double* coef, int64_t nside, int64_t lmax, int64_t spin, 
                  const double* gamma, const double* seed_mant, const int32_t* seed_exp,
                  const int32_t* nactive, double* spec, int64_t ld, void* stream);
 
-/* Number of ring chunks the analysis kernels split the rings of one order into. */
+/* Partial slots of one analysis pass (FP64 kernels), and the number of ring chunks the
+ * analysis kernels split the rings of one order into (mixed != 0: the mixed kernels with
+ * nmaps maps per recurrence); passes start at chunk0 = 0, slots, 2 slots, ... */
 int64_t gm_lat_nchunks(int64_t nside, int64_t spin);
+int64_t gm_lat_ring_chunks(int64_t nside, int64_t spin, int64_t mixed, int64_t nmaps);
 
 /* ---------------------------------------------------------------- power spectra, MCM */
 
@@ -215,18 +225,19 @@ int gm_alm_filter(const double* alm, int64_t nin, int64_t alm_stride, const doub
  * and for spin 0 depend on the number of maps sharing the recurrence: 1, 2, 4 or 8). */
 int64_t gm_lat_nchunks_mixed(int64_t nside, int64_t spin, int64_t nmaps);
 int gm_lat_finish_s0_mixed(const double* partial, int64_t nmaps, int64_t nside, const int32_t* nactive,
-                           const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode, void* stream);
+                           const double* P, int64_t lmax, double* alm, int64_t alm_map_stride, int mode,
+                           int64_t chunk0, void* stream);
 int gm_lat_finish_s_mixed(const double* partial, int64_t nside, const int32_t* nactive, const double* P, int64_t lmax,
-                          int64_t spin, double* alm_e, double* alm_b, int mode, void* stream);
+                          int64_t spin, double* alm_e, double* alm_b, int mode, int64_t chunk0, void* stream);
 int gm_lat_ana_s0_mixed(const double* spec, int64_t nmaps, int64_t ld, int64_t smap, int64_t nside, int64_t lmax,
                         const double* alpha, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
-                        double* partial, int64_t nchunks, void* stream);
+                        double* partial, int64_t nchunks, int64_t chunk0, void* stream);
 int gm_lat_syn_s0_mixed(const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
                         const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, double* spec,
                         int64_t ld, int64_t smap, void* stream);
 int gm_lat_ana_s_mixed(const double* spec, int64_t ld, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
                        const double* gamma, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
-                       double* partial, int64_t nchunks, void* stream);
+                       double* partial, int64_t nchunks, int64_t chunk0, void* stream);
 int gm_lat_syn_s_mixed(const double* coef, int64_t nside, int64_t lmax, int64_t spin, const double* alpha,
                        const double* gamma, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
                        double* spec, int64_t ld, void* stream);

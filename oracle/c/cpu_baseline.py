@@ -33,6 +33,8 @@ def load():
     p = ctypes.c_void_p
     lib.gmo_lat_ana_s.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, p, ctypes.c_int64, p, ctypes.c_int, p]
     lib.gmo_lat_ana_s.restype = None
+    lib.gmo_lat_ana_s_cols.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, p, p, p, ctypes.c_int, p]
+    lib.gmo_lat_ana_s_cols.restype = None
     lib.gmo_coupling.argtypes = [ctypes.c_int, p, ctypes.c_int, p, p, ctypes.c_int, p]
     lib.gmo_coupling.restype = None
     return lib
@@ -47,6 +49,22 @@ def lat_ana_s(nside, lmax, spin, spec, orders):
     out = np.zeros((len(orders), L, 4))
     lib.gmo_lat_ana_s(nside, lmax, spin, spec.ctypes.data, spec.shape[1], orders.ctypes.data, len(orders),
                       out.ctypes.data)
+    return out
+
+
+def lat_ana_s_cols(nside, lmax, spin, dcol, mcol, orders):
+    """As ``lat_ana_s`` from per-order spectrum columns ``dcol[k][r]`` (order +m) and
+    ``mcol[k][r]`` (order -m), each ``(len(orders), nring)`` complex."""
+    lib = load()
+    dcol = np.ascontiguousarray(dcol, dtype=np.complex128)
+    mcol = np.ascontiguousarray(mcol, dtype=np.complex128)
+    orders = np.ascontiguousarray(orders, dtype=np.int32)
+    if dcol.shape != (len(orders), 4 * nside - 1) or mcol.shape != dcol.shape:
+        raise ValueError("columns must be (len(orders), nring)")
+    L = lmax + 1
+    out = np.zeros((len(orders), L, 4))
+    lib.gmo_lat_ana_s_cols(nside, lmax, spin, dcol.ctypes.data, mcol.ctypes.data, orders.ctypes.data, len(orders),
+                           out.ctypes.data)
     return out
 
 

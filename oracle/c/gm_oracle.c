@@ -52,73 +52,93 @@ static double seed_log2_const(int m, int s) {
     return 0.5 * log2((2.0 * M + 1.0) / (4.0 * M_PI)) + 0.5 * lb;
 }
 
-void gmo_lat_ana_s(int nside, int lmax, int spin, const double* spec, int64_t ld, const int* orders,
-                   int norders, double* out) {
+/* One order m: acc[4 l + (dre, dim, mre, mim)] = sum_r lambda_lm(theta_r) (direct, mirror),
+ * the direct value of ring r at dbase + 2 r dstride, the mirror value of ring r (the
+ * spectrum at -m of the ring reflected through the equator) at mbase + 2 r mstride. */
+static void ana_one(int nside, int lmax, int s, int m, const double* dbase, int64_t dstride, const double* mbase,
+                    int64_t mstride, double* acc) {
     const int L = lmax + 1;
     const int nring = 4 * nside - 1;
-    const int s = spin;
-#pragma omp parallel for schedule(dynamic, 1)
-    for (int io = 0; io < norders; ++io) {
-        const int m = orders[io];
-        const int M = m > s ? m : s;
-        double* acc = out + (int64_t)io * L * 4;
-        for (int l = 0; l < 4 * L; ++l) acc[l] = 0.0;
-        if (M >= L) continue;
-        double* A = (double*)malloc(sizeof(double) * 3 * L);
-        double* C = A + L;
-        double* B = A + 2 * L;
-        for (int lp = M + 1; lp < L; ++lp) {
-            const int l = lp - 1;
-            const double K = (l + 1.0) * (2.0 * l + 1.0) /
-                             sqrt(((l + 1.0) * (l + 1.0) - (double)m * m) * ((l + 1.0) * (l + 1.0) - (double)s * s));
-            A[lp] = sqrt((2.0 * l + 3.0) / (2.0 * l + 1.0)) * K;
-            C[lp] = l > 0 ? A[lp] * m * s / ((double)l * (l + 1.0)) : 0.0;
-            B[lp] = l > 0 ? sqrt((2.0 * l + 3.0) / (2.0 * l - 1.0)) * K *
-                                sqrt(((double)l * l - (double)m * m) * ((double)l * l - (double)s * s)) /
-                                ((double)l * (2.0 * l + 1.0))
-                          : 0.0;
-        }
-        const double c0 = seed_log2_const(m, s);
-        const double sign = ((m + s) & 1) ? -1.0 : 1.0;
-        for (int r = 0; r < nring; ++r) {
-            const ring_t g = ring_geom(nside, r);
-            const int rm = nring - 1 - r;
-            const double* gd = spec + 2 * ((int64_t)r * ld + (L - 1) + m);
-            const double* gm = spec + 2 * ((int64_t)rm * ld + (L - 1) - m);
-            const double dre = gd[0], dim = gd[1];
-            const double mre = m ? gm[0] : 0.0, mim = m ? gm[1] : 0.0;
-            const double f = c0 + (m + s) * log2(g.shalf) + abs(m - s) * log2(g.chalf);
-            int e = (int)floor(f);
-            double v = sign * exp2(f - e), vp = 0.0;
-            double scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
-            for (int l = M; l < L; ++l) {
-                if (scale != 0.0) {
-                    const double lam = v * scale;
-                    acc[4 * l + 0] += lam * dre;
-                    acc[4 * l + 1] += lam * dim;
-                    acc[4 * l + 2] += lam * mre;
-                    acc[4 * l + 3] += lam * mim;
-                }
-                if (l + 1 < L) {
-                    const double vn = (A[l + 1] * g.z + C[l + 1]) * v - B[l + 1] * vp;
-                    vp = v;
-                    v = vn;
-                    if (((l - M) & 7) == 7) {
-                        const double big = fabs(v) > fabs(vp) ? fabs(v) : fabs(vp);
-                        if (big > 0x1p256 || (big < 0x1p-256 && big > 0)) {
-                            int t;
-                            frexp(big, &t);
-                            v = ldexp(v, -t);
-                            vp = ldexp(vp, -t);
-                            e += t;
-                            scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
-                        }
+    const int M = m > s ? m : s;
+    for (int l = 0; l < 4 * L; ++l) acc[l] = 0.0;
+    if (M >= L) return;
+    double* A = (double*)malloc(sizeof(double) * 3 * L);
+    double* C = A + L;
+    double* B = A + 2 * L;
+    for (int lp = M + 1; lp < L; ++lp) {
+        const int l = lp - 1;
+        const double K = (l + 1.0) * (2.0 * l + 1.0) /
+                         sqrt(((l + 1.0) * (l + 1.0) - (double)m * m) * ((l + 1.0) * (l + 1.0) - (double)s * s));
+        A[lp] = sqrt((2.0 * l + 3.0) / (2.0 * l + 1.0)) * K;
+        C[lp] = l > 0 ? A[lp] * m * s / ((double)l * (l + 1.0)) : 0.0;
+        B[lp] = l > 0 ? sqrt((2.0 * l + 3.0) / (2.0 * l - 1.0)) * K *
+                            sqrt(((double)l * l - (double)m * m) * ((double)l * l - (double)s * s)) /
+                            ((double)l * (2.0 * l + 1.0))
+                      : 0.0;
+    }
+    const double c0 = seed_log2_const(m, s);
+    const double sign = ((m + s) & 1) ? -1.0 : 1.0;
+    for (int r = 0; r < nring; ++r) {
+        const ring_t g = ring_geom(nside, r);
+        const int rm = nring - 1 - r;
+        const double* gd = dbase + 2 * (int64_t)r * dstride;
+        const double* gm = mbase + 2 * (int64_t)rm * mstride;
+        const double dre = gd[0], dim = gd[1];
+        const double mre = m ? gm[0] : 0.0, mim = m ? gm[1] : 0.0;
+        const double f = c0 + (m + s) * log2(g.shalf) + abs(m - s) * log2(g.chalf);
+        int e = (int)floor(f);
+        double v = sign * exp2(f - e), vp = 0.0;
+        double scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
+        for (int l = M; l < L; ++l) {
+            if (scale != 0.0) {
+                const double lam = v * scale;
+                acc[4 * l + 0] += lam * dre;
+                acc[4 * l + 1] += lam * dim;
+                acc[4 * l + 2] += lam * mre;
+                acc[4 * l + 3] += lam * mim;
+            }
+            if (l + 1 < L) {
+                const double vn = (A[l + 1] * g.z + C[l + 1]) * v - B[l + 1] * vp;
+                vp = v;
+                v = vn;
+                if (((l - M) & 7) == 7) {
+                    const double big = fabs(v) > fabs(vp) ? fabs(v) : fabs(vp);
+                    if (big > 0x1p256 || (big < 0x1p-256 && big > 0)) {
+                        int t;
+                        frexp(big, &t);
+                        v = ldexp(v, -t);
+                        vp = ldexp(vp, -t);
+                        e += t;
+                        scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
                     }
                 }
             }
         }
-        free(A);
     }
+    free(A);
+}
+
+void gmo_lat_ana_s(int nside, int lmax, int spin, const double* spec, int64_t ld, const int* orders,
+                   int norders, double* out) {
+    const int L = lmax + 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int io = 0; io < norders; ++io) {
+        const int m = orders[io];
+        ana_one(nside, lmax, spin, m, spec + 2 * ((int64_t)(L - 1) + m), ld, spec + 2 * ((int64_t)(L - 1) - m), ld,
+                out + (int64_t)io * L * 4);
+    }
+}
+
+/* The same from per-order columns: dcol[io][r] = spec[r][L-1+m], mcol[io][r] = spec[r][L-1-m]
+ * (complex, nring per order), for geometries whose full spectrum is too large for the host. */
+void gmo_lat_ana_s_cols(int nside, int lmax, int spin, const double* dcol, const double* mcol, const int* orders,
+                        int norders, double* out) {
+    const int L = lmax + 1;
+    const int64_t nring = 4 * (int64_t)nside - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int io = 0; io < norders; ++io)
+        ana_one(nside, lmax, spin, orders[io], dcol + 2 * io * nring, 1, mcol + 2 * io * nring, 1,
+                out + (int64_t)io * L * 4);
 }
 
 void gmo_coupling(int lmax, const double* W, int lmax_w, const double* g, const int* rows, int nrows,

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _lib.query("gm_abi_version") == 1
+    assert _lib.query("gm_abi_version") == 2
 
 
 def test_table_size_matches_layout():
@@ -43,6 +43,18 @@ def test_nchunks():
     assert _lib.query("gm_lat_nchunks", 64, 0) == 1
     assert _lib.query("gm_lat_nchunks", 4096, 0) == 6
     assert _lib.query("gm_lat_nchunks", 4096, 2) == 8
+    # one analysis pass over every ring chunk up to Nside 4096 ...
+    for spin in (0, 2):
+        for mixed in (0, 1):
+            slots = _lib.query("gm_lat_nchunks_mixed" if mixed else "gm_lat_nchunks", *((4096, spin, 1) if mixed else (4096, spin)))
+            assert _lib.query("gm_lat_ring_chunks", 4096, spin, mixed, 1) == slots
+    # ... and passes of 4 slots above (Nside 8192: 32767 rings in chunks of 2048 / 1536)
+    assert _lib.query("gm_lat_ring_chunks", 8192, 2, 0, 1) == 16
+    assert _lib.query("gm_lat_nchunks", 8192, 2) == 4
+    assert _lib.query("gm_lat_ring_chunks", 8192, 2, 1, 1) == 22
+    assert _lib.query("gm_lat_nchunks_mixed", 8192, 2, 1) == 4
+    assert _lib.query("gm_lat_ring_chunks", 8192, 0, 0, 2) == 11
+    assert _lib.query("gm_lat_nchunks", 8192, 0) == 4
 
 
 @pytest.mark.parametrize(
@@ -52,9 +64,11 @@ def test_nchunks():
         ("gm_ring_fft_fwd_s0", (1, 1, 768, 8, 10, 1, 5, 0, None), 2),     # ld < lmax + 1
         ("gm_ring_fft_fwd_s0", (1, 1, 0, 16384, 10, 1, 11, 0, None), 3),  # nside above the FFT limit
         ("gm_ring_fft_fwd_s", (1, 1, 8, 10, 1, 20, None), 2),            # ld < 2 lmax + 1
-        ("gm_lat_ana_s0", (1, 3, 11, 0, 8, 10, 1, 1, 1, 1, 1, 1, None), 1),  # nmaps = 3
-        ("gm_lat_ana_s0", (1, 1, 11, 0, 8, 10, 1, 1, 1, 1, 1, 7, None), 2),  # wrong chunk count
-        ("gm_lat_ana_s", (1, 21, 8, 10, 0, 1, 1, 1, 1, 1, 1, 1, None), 1),   # spin 0 on the spin-s entry
+        ("gm_lat_ana_s0", (1, 3, 11, 0, 8, 10, 1, 1, 1, 1, 1, 1, 0, None), 1),  # nmaps = 3
+        ("gm_lat_ana_s0", (1, 1, 11, 0, 8, 10, 1, 1, 1, 1, 1, 7, 0, None), 2),  # wrong chunk count
+        ("gm_lat_ana_s0", (1, 1, 11, 0, 8, 10, 1, 1, 1, 1, 1, 1, 1, None), 1),  # chunk0 past the last chunk
+        ("gm_lat_ana_s", (1, 21, 8, 10, 0, 1, 1, 1, 1, 1, 1, 1, 0, None), 1),   # spin 0 on the spin-s entry
+        ("gm_lat_finish_s0", (1, 1, 8, 1, 1, 10, 1, 66, 3, 0, None), 1),       # mode 3
         ("gm_lat_tables", (10, 11, 1, 1, 1, None), 1),                      # spin > lmax
         ("gm_coupling", (None, 10, 1, 10, 1, None, None, None, None), 1),   # null window
         ("gm_coupling", (1, 21, 1, 10, 1, None, None, None, None), 1),      # window beyond 2 lmax

@@ -60,3 +60,21 @@ def test_n3j_count():
 
     assert n3j(3) == sum(min(a + b, 3) - (b - a) + 1 for a in range(4) for b in range(a, 4))
     assert abs(n3j(12287) - 4.64e11) / 4.64e11 < 0.01
+
+
+def test_column_entry_matches_full_spectrum_entry():
+    """``lat_ana_s_cols`` (per-order columns, for spectra too large for the host) gives
+    exactly what ``lat_ana_s`` gives from the full spectrum."""
+    import subprocess
+    import os
+    subprocess.run(["make", "-s", "-C", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                      "oracle", "c")], check=True)
+    from oracle.c import cpu_baseline as cb
+    nside, lmax, spin = 8, 20, 2
+    nring, L = 4 * nside - 1, lmax + 1
+    rng = np.random.default_rng(3)
+    spec = rng.normal(size=(nring, 2 * L - 1)) + 1j * rng.normal(size=(nring, 2 * L - 1))
+    orders = np.array([0, 1, 5, 13, 20])
+    full = cb.lat_ana_s(nside, lmax, spin, spec, orders)
+    cols = cb.lat_ana_s_cols(nside, lmax, spin, spec[:, L - 1 + orders].T, spec[:, L - 1 - orders].T, orders)
+    assert np.array_equal(full, cols)
