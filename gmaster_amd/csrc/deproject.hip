@@ -1,7 +1,8 @@
-// Contaminant deprojection on gfx950, FP64: weighted map inner products (the template
-// covariance and the template-map products), the subtraction of the best-fit templates,
-// and the per-multipole filter of alms by a matrix of spectra.  All three are HBM-bound
-// streams over (nmaps x npix) maps or healpy-packed alms.
+// Contaminant deprojection and E/B purification on gfx950, FP64: weighted map inner
+// products (the template covariance and the template-map products), the subtraction of
+// the best-fit templates, the per-multipole filter of alms by a matrix of spectra, and the
+// spin-weighted map products of the purification.  All are HBM-bound streams over
+// (nmaps x npix) maps or healpy-packed alms.
 #include "common.h"
 
 namespace gm {
@@ -136,4 +137,31 @@ extern "C" int gm_alm_filter(const double* alm, int64_t nin, int64_t alm_stride,
                        reinterpret_cast<const double2*>(alm), (int)nin, alm_stride, cl, (int)nout, (int)L,
                        reinterpret_cast<double2*>(out), out_stride);
     return check_launch("gm_alm_filter");
+}
+
+// ------------------------------------------------------------------------ purification
+namespace gm {
+namespace {
+
+// out = conj(w) p per pixel for two-component (re, im) maps: (w0 p0 + w1 p1, w0 p1 - w1 p0)
+__global__ void k_spin_mul(const double* __restrict__ w, const double* __restrict__ p, int64_t npix,
+                           double* __restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += stride) {
+        const double w0 = w[i], w1 = w[npix + i], p0 = p[i], p1 = p[npix + i];
+        out[i] = w0 * p0 + w1 * p1;
+        out[npix + i] = w0 * p1 - w1 * p0;
+    }
+}
+
+}  // namespace
+}  // namespace gm
+
+extern "C" int gm_spin_mul(const double* w, const double* p, int64_t npix, double* out, void* stream) {
+    if (!w || !p || !out) return fail(GM_E_ARG, "gm_spin_mul: null pointer");
+    if (npix < 1) return fail(GM_E_ARG, "gm_spin_mul: npix must be positive");
+    int64_t blocks = (npix + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_spin_mul, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, w, p, npix, out);
+    return check_launch("gm_spin_mul");
 }

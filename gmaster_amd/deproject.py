@@ -8,8 +8,8 @@
   host (``moore_penrose_pinvh``, ``gmaster/utils.py:527-553``).
 * ``filter_alms`` — ``gm_alm_filter`` (``gmaster/workspaces.py:2258-2265``).
 * ``deprojection_bias``, ``uncorr_noise_deprojection_bias``, ``compute_full_master`` —
-  ``gmaster/workspaces.py:2268-2520`` for non-purified fields, composed from the engine's
-  transforms, ``gm_coupled_cell`` and the primitives above.
+  ``gmaster/workspaces.py:2268-2520``, composed from the engine's transforms (purified for
+  pure fields), ``gm_coupled_cell`` and the primitives above.
 """
 
 import numpy as np
@@ -105,9 +105,21 @@ def _iM(field):
     return torch.as_tensor(field.iM, dtype=torch.float64)
 
 
+def _transformed(field, maps, n_iter):
+    """The field's alms of unmasked ``maps``: masked and analysed, or purified
+    (``transformed`` of ``gmaster/workspaces.py:2297-2310``)."""
+    lmax = field.ainfo.lmax
+    if field.pure_e or field.pure_b:
+        from .purify import purify
+        amask = field.get_mask_alms().as_subclass(torch.Tensor)
+        return purify(field.mask, amask, maps, lmax, (field.pure_e, field.pure_b), n_iter, precision=_precision(),
+                      return_maps=False)
+    return _map2alm(maps * field.mask[None, :], field.spin, lmax, n_iter)
+
+
 def deprojection_bias(f1, f2, cl_guess, n_iter=None):
     """Coupled bias of the pseudo-spectrum from template deprojection, ``(n1 n2, lmax+1)``
-    (``gmaster/workspaces.py:2268-2386``; fields without purification)."""
+    (``gmaster/workspaces.py:2268-2386``)."""
     if not f1.is_compatible(f2):
         raise ValueError("Fields have incompatible pixelizations")
     lmax = f1.ainfo.lmax
@@ -126,7 +138,7 @@ def deprojection_bias(f1, f2, cl_guess, n_iter=None):
         for tj in f1.temp:
             alms = filter_alms(_map2alm(tj * f1.mask[None, :], f1.spin, lmax, n_iter), spectra, lmax)
             maps = _alm2map(alms, f2.spin, nside, lmax)
-            filtered = _map2alm(maps * f2.mask[None, :], f2.spin, lmax, n_iter)
+            filtered = _transformed(f2, maps, n_iter)
             ff.append(torch.stack([_spectra(ti, filtered, lmax) for ti in f1.alm_temp]).cpu())
         ff = torch.stack(ff, dim=1)
         bias -= torch.einsum("ij,ijklm->klm", _iM(f1), ff)
@@ -139,7 +151,7 @@ def deprojection_bias(f1, f2, cl_guess, n_iter=None):
             masked = maps * f1.mask[None, :]
             if f1.n_temp:
                 products.append(map_dots(f1.temp, masked[None])[:, 0].cpu())
-            filtered = _map2alm(masked, f1.spin, lmax, n_iter)
+            filtered = _transformed(f1, maps, n_iter)
             gg.append(torch.stack([_spectra(filtered, ti, lmax) for ti in f2.alm_temp]).cpu())
         gg = torch.stack(gg, dim=1)
         bias -= torch.einsum("ij,ijklm->klm", _iM(f2), gg)

@@ -5,15 +5,17 @@ A field masks its maps, subtracts the best fit of its (masked) contaminant templ
 on the host, subtraction by ``gm_map_deproject``) and analyses the result (``n_iter``
 Jacobi steps).  A spin-0 field whose mask has the same band limit and ``n_iter`` analyses
 the masked map and the mask in one shared latitudinal pass (``gmaster/field.py:273-289``);
-other fields compute their mask alms lazily (``:348-376``).  Purification and anisotropic
-masks are outside this engine and raise ``NotImplementedError``.
+other fields compute their mask alms lazily (``:348-376``).  Spin-2 fields may be E- and/or
+B-purified (``gmaster/field.py:433-499``, ``gmaster_amd.purify``); their maps are then the
+purified ones.  Anisotropic masks are outside this engine and raise ``NotImplementedError``.
 """
 
 import torch
 
-from . import sht
+from . import _lib, sht
 from ._array import device_tensor, wrap
 from .deproject import deproject
+from .purify import purify
 from ._config import nmt_params
 from .utils import NmtAlmInfo, NmtMapInfo
 
@@ -48,14 +50,12 @@ class NmtField:
     def __init__(self, mask, maps, *, spin=None, templates=None, beam=None, purify_e=False, purify_b=False,
                  n_iter=None, n_iter_mask=None, tol_pinv=None, wcs=None, lmax=None, lmax_mask=None,
                  masked_on_input=False, lite=False, mask_22=None, mask_12=None):
-        if purify_e or purify_b:
-            raise NotImplementedError("E/B purification is not supported by this engine")
         if mask_22 is not None or mask_12 is not None:
             raise NotImplementedError("anisotropic masks are not supported by this engine")
         self.lite = bool(lite)
         self.n_iter = nmt_params.n_iter_default if n_iter is None else int(n_iter)
         self.n_iter_mask = nmt_params.n_iter_mask_default if n_iter_mask is None else int(n_iter_mask)
-        self.pure_e = self.pure_b = False
+        self.pure_e, self.pure_b = bool(purify_e), bool(purify_b)
         self.is_catalog = False
         self.anisotropic_mask = False
         self.alm = self.alm_mask = None
@@ -107,6 +107,17 @@ class NmtField:
             raise ValueError("Spin-zero fields are associated with a single map")
         self.spin = int(spin)
         self.nmaps = 1 if self.spin == 0 else 2
+        pure_any = self.pure_e or self.pure_b
+        if pure_any and self.spin != 2:
+            raise ValueError("Purification is only defined for spin-2 fields")
+        if pure_any and self.ainfo != self.ainfo_mask:
+            raise ValueError("Pure fields require lmax == lmax_mask")
+        unmasked = None
+        if pure_any:
+            unmasked = maps_t.clone()
+            if masked_on_input:
+                good = self.mask > 0
+                unmasked = torch.where(good[None, :], maps_t / torch.where(good, self.mask, 1.0)[None, :], maps_t)
 
         masked = maps_t if masked_on_input else maps_t * self.mask[None, :]
         temps = None
@@ -119,11 +130,27 @@ class NmtField:
             if masked is maps_t:
                 masked = masked.clone()  # the subtraction is in place; keep the caller's maps
             tol = nmt_params.tol_pinv_default if tol_pinv is None else float(tol_pinv)
+            temps_unmasked = None
+            if pure_any:
+                temps_unmasked = device_tensor(templates, device=dev)
+                if masked_on_input:
+                    good = self.mask > 0
+                    temps_unmasked = torch.where(good[None, None, :],
+                                                 temps_unmasked / torch.where(good, self.mask, 1.0)[None, None, :],
+                                                 temps_unmasked)
             self.alphas, self.iM = deproject(masked, temps, tol)
             self.n_temp = len(temps)
+            if pure_any:
+                alpha_d = torch.as_tensor(self.alphas, dtype=torch.float64, device=dev)
+                _lib.call("gm_map_deproject", _lib.ptr(unmasked), unmasked.numel(), _lib.ptr(temps_unmasked),
+                          len(temps), unmasked.numel(), _lib.ptr(alpha_d), _lib.stream_ptr(dev))
         key = _mask_key(self.mask, lmax_mask, self.n_iter_mask)
         self.alm_mask = _MASK_ALMS.get(key)
-        if (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
+        if pure_any:
+            amask = self.get_mask_alms().as_subclass(torch.Tensor)
+            self.alm, masked = purify(self.mask, amask, unmasked, lmax, (self.pure_e, self.pure_b), self.n_iter,
+                                      precision=_precision())
+        elif (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
                 and self.n_iter == self.n_iter_mask and self.alm_mask is None):
             pair = sht.map2alm_s0(torch.cat([masked, self.mask[None, :]]), lmax, n_iter=self.n_iter,
                                   precision=_precision())
@@ -136,7 +163,12 @@ class NmtField:
             self.maps = masked
             if temps is not None:
                 self.temp = temps
-                if self.spin == 0:
+                if pure_any:
+                    amask = self.get_mask_alms().as_subclass(torch.Tensor)
+                    self.alm_temp = torch.stack([
+                        purify(self.mask, amask, t, lmax, (self.pure_e, self.pure_b), self.n_iter,
+                               precision=_precision(), return_maps=False) for t in temps_unmasked])
+                elif self.spin == 0:
                     self.alm_temp = sht.map2alm_s0(temps[:, 0, :], lmax, n_iter=self.n_iter,
                                                    precision=_precision())[:, None, :]
                 else:

@@ -323,6 +323,9 @@ class NmtWorkspace:
         for f in (fl1, fl2):
             if f.spin not in (0, 2):
                 raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
+            if f.pure_e or f.pure_b:
+                raise NotImplementedError("the pure-E/B coupling blocks are not built; purified fields give "
+                                          "alms, maps and coupled spectra only")
         self.spin1, self.spin2 = fl1.spin, fl2.spin
         self.nmaps1 = 1 if self.spin1 == 0 else 2
         self.nmaps2 = 1 if self.spin2 == 0 else 2

@@ -209,6 +209,12 @@ int gm_map_dots(const double* a, int64_t na, int64_t lda, const double* b, int64
 int gm_map_deproject(double* maps, int64_t n, const double* templates, int64_t ntemp, int64_t ldt,
                      const double* alphas, void* stream);
 
+/* Product of two-component maps, out = conj(w) p per pixel:
+ *   out[0] = w[0] p[0] + w[1] p[1],  out[1] = w[0] p[1] - w[1] p[0]
+ * (w, p, out: 2 x npix; out may alias p).  The mask-derivative products of the E/B
+ * purification, gmaster/field.py:458-462 and :473-476. */
+int gm_spin_mul(const double* w, const double* p, int64_t npix, double* out, void* stream);
+
 /* out[o] = sum_k cl[k][o][l] alm[k] per healpy-packed element (l, m): cl is nin x nout x
  * (lmax+1) doubles, alm nin and out nout complex arrays with the given strides (complex
  * elements); out must not alias alm.  Replaces _filter_alms (gmaster/workspaces.py:2258-2265). */

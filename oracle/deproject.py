@@ -56,21 +56,43 @@ def filter_alms(alms, spectra, lmax):
 
 
 class Field:
-    """Non-purified field with templates: the pieces the bias formulas read."""
+    """Field with templates, optionally E/B-purified (``task``): the pieces the bias
+    formulas read (``gmaster/field.py:193-315``, not pre-masked)."""
 
-    def __init__(self, mask, maps, templates, spin, nside, lmax, n_iter, tol=1e-10):
+    def __init__(self, mask, maps, templates, spin, nside, lmax, n_iter, tol=1e-10, task=(False, False)):
+        from .purify import purify
         self.mask = np.asarray(mask, dtype=np.float64)
         self.spin, self.nside, self.lmax, self.n_iter = spin, nside, lmax, n_iter
         self.nmaps = 1 if spin == 0 else 2
+        self.task = tuple(task)
+        self.pure = any(self.task)
+        unmasked = np.atleast_2d(np.asarray(maps, dtype=np.float64)).copy()
         if templates is None:
-            self.maps = np.atleast_2d(maps) * self.mask[None, :]
+            self.maps = unmasked * self.mask[None, :]
             self.temp, self.iM, self.alphas = None, None, None
             self.n_temp = 0
         else:
             self.maps, self.temp, self.alphas, self.iM = deproject(self.mask, maps, templates, tol)
             self.n_temp = len(self.temp)
-            self.alm_temp = np.array([sht.map2alm(t, spin, nside, lmax, n_iter) for t in self.temp])
-        self.alm = sht.map2alm(self.maps, spin, nside, lmax, n_iter)
+            unmasked = unmasked - np.einsum("i,iap->ap", self.alphas, np.asarray(templates, dtype=np.float64))
+        if self.pure:
+            self.alm_mask = sht.map2alm(self.mask[None], 0, nside, lmax, n_iter)[0]
+            self.alm = purify(self.mask, self.alm_mask, unmasked, nside, lmax, self.task, n_iter)
+            self.maps = sht.alm2map(self.alm, 2, nside, lmax)
+            if self.n_temp:
+                self.alm_temp = np.array([purify(self.mask, self.alm_mask, t, nside, lmax, self.task, n_iter)
+                                          for t in np.asarray(templates, dtype=np.float64)])
+        else:
+            self.alm = sht.map2alm(self.maps, spin, nside, lmax, n_iter)
+            if self.n_temp:
+                self.alm_temp = np.array([sht.map2alm(t, spin, nside, lmax, n_iter) for t in self.temp])
+
+    def transformed(self, maps):
+        """Alms of unmasked maps as the field forms its own (masked, or purified)."""
+        from .purify import purify
+        if self.pure:
+            return purify(self.mask, self.alm_mask, maps, self.nside, self.lmax, self.task, self.n_iter)
+        return sht.map2alm(maps * self.mask[None, :], self.spin, self.nside, self.lmax, self.n_iter)
 
 
 def _spectra(first, second, lmax):
@@ -89,7 +111,7 @@ def deprojection_bias(f1, f2, cl_guess):
             alms = sht.map2alm(tj * f1.mask[None, :], f1.spin, nside, lmax, n_iter)
             alms = filter_alms(alms, spectra, lmax)
             maps = sht.alm2map(alms, f2.spin, nside, lmax)
-            filtered = sht.map2alm(maps * f2.mask[None, :], f2.spin, nside, lmax, n_iter)
+            filtered = f2.transformed(maps)
             ff.append(np.array([_spectra(ti, filtered, lmax) for ti in f1.alm_temp]))
         ff = np.stack(ff, axis=1)
         bias -= np.einsum("ij,ijklm->klm", f1.iM, ff)
@@ -101,7 +123,7 @@ def deprojection_bias(f1, f2, cl_guess):
             maps = sht.alm2map(alms, f1.spin, nside, lmax)
             if f1.n_temp:
                 products.append(np.array([w * np.sum(ti * maps * f1.mask[None, :]) for ti in f1.temp]))
-            filtered = sht.map2alm(maps * f1.mask[None, :], f1.spin, nside, lmax, n_iter)
+            filtered = f1.transformed(maps)
             gg.append(np.array([_spectra(filtered, ti, lmax) for ti in f2.alm_temp]))
         gg = np.stack(gg, axis=1)
         bias -= np.einsum("ij,ijklm->klm", f2.iM, gg)
