@@ -60,6 +60,11 @@ SIGNATURES = {
     "gm_map_deproject": [_p, _i64, _p, _i64, _i64, _p, _p],
     "gm_alm_filter": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p],
     "gm_spin_mul": [_p, _p, _i64, _p, _p],
+    "gm_gauss_legendre": [_i64, _p, _p, _p],
+    "gm_wigner_d_table": [_p, _i64, _i64, _i64, _i64, _p, _p, _p],
+    "gm_wigner_d_corr": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p],
+    "gm_table_combine": [_i64, _p, _p, _p, _p, _p, _i64, _i64, _p, _p],
+    "gm_parity_split": [_p, _p, _i64, _p, _p, _p],
     "gm_table_size": [_i64],
     "gm_lat_nchunks": [_i64, _i64],
     "gm_lat_nchunks_mixed": [_i64, _i64, _i64],

@@ -175,23 +175,35 @@ def _coupled_cell(alm1, alm2, lmax):
     return cl
 
 
-def _layout(ncls, spin1, spin2):
+_TE_LEVELS = ("02", "02pure")
+_EVEN_LEVELS = ("22p", "22p1", "22p2")
+_ODD_LEVELS = ("22m", "22m1", "22m2")
+
+
+def _layout(ncls, spin1, spin2, purity=(False, False, False, False)):
     """``[(block name, (row channel, column channel), sign)]`` (``gmaster/workspaces.py:1851-1903``).
     ncls 7 is the joint TEB layout: TT, then TE/TB with sign (-1)^spin2, then the spin x spin
-    slots at offset 3 with the M+ / M- signs of spin_sign = 1."""
+    slots at offset 3 with the M+ / M- signs of spin_sign = 1.  With purification
+    ``(pure_e1, pure_b1, pure_e2, pure_b2)`` a slot takes the block of its level, the number
+    of purified fields among the slot's two channels (``:1845-1850, 1865-1901``; ncls 7 uses
+    the spin-2 field's flags for both channels)."""
+    pe1, pb1, pe2, pb2 = (int(bool(v)) for v in purity)
     if ncls == 1:
         return [("00", (0, 0), 1.0)]
+    if ncls == 2:
+        sign = float((-1) ** (spin1 + spin2))
+        return [(_TE_LEVELS[pe1 + pe2], (0, 0), sign), (_TE_LEVELS[pb1 + pb2], (1, 1), sign)]
     if ncls == 7:
         out = [("00", (0, 0), 1.0)]
-        out += [("02", (c, c), float((-1) ** spin2)) for c in (1, 2)]
-        out += [("22p", (3 + c, 3 + c), 1.0) for c in range(4)]
-        out += [("22m", (3 + c, 6 - c), -1.0 if c in (1, 2) else 1.0) for c in range(4)]
-        return out
-    sign = float((-1) ** (spin1 + spin2))
-    if ncls == 2:
-        return [("02", (0, 0), sign), ("02", (1, 1), sign)]
-    out = [("22p", (c, c), sign) for c in range(4)]
-    out += [("22m", (c, 3 - c), -sign if c in (1, 2) else sign) for c in range(4)]
+        out += [(_TE_LEVELS[lv], (1 + c, 1 + c), float((-1) ** spin2)) for c, lv in enumerate((pe2, pb2))]
+        pe1, pb1 = pe2, pb2
+        offset, sign = 3, 1.0
+    else:
+        out, offset, sign = [], 0, float((-1) ** (spin1 + spin2))
+    levels = (pe1 + pe2, pe1 + pb2, pb1 + pe2, pb1 + pb2)
+    out += [(_EVEN_LEVELS[lv], (offset + c, offset + c), sign) for c, lv in enumerate(levels)]
+    out += [(_ODD_LEVELS[lv], (offset + c, offset + 3 - c), -sign if c in (1, 2) else sign)
+            for c, lv in enumerate(levels)]
     return out
 
 
@@ -234,8 +246,9 @@ class NmtWorkspace:
         header = {"EXTNAME": "WSP_PRIMARY", "LMAX": self.lmax, "LMAX_MASK": self.lmax_mask,
                   "IS_TEB": bool(getattr(self, "is_teb", False)),
                   "NCLS": self.ncls, "NORM_TYPE": self.norm_type, "WAWB": float(self.wawb), "SPIN1": self.spin1,
-                  "SPIN2": self.spin2, "ANISO1": False, "ANISO2": False, "PURE_E1": False, "PURE_B1": False,
-                  "PURE_E2": False, "PURE_B2": False, "L_TOEPLITZ": tl, "L_EXACT": te, "DL_BAND": db}
+                  "SPIN2": self.spin2, "ANISO1": False, "ANISO2": False, "PURE_E1": bool(self.purity[0]),
+                  "PURE_B1": bool(self.purity[1]), "PURE_E2": bool(self.purity[2]), "PURE_B2": bool(self.purity[3]),
+                  "L_TOEPLITZ": tl, "L_EXACT": te, "DL_BAND": db}
         L = self.lmax + 1
         ell = np.arange(L, dtype=np.int32)
         b = self.bins
@@ -262,8 +275,9 @@ class NmtWorkspace:
         hdus = fits.read(fname)
         h, dense = fits.find(hdus, "WSP_PRIMARY")
         self.is_teb = bool(h.get("IS_TEB", False))
-        if any(bool(h.get(k, False)) for k in ("PURE_E1", "PURE_B1", "PURE_E2", "PURE_B2", "ANISO1", "ANISO2")):
-            raise NotImplementedError("purified or anisotropic workspaces are not supported by this engine")
+        if any(bool(h.get(k, False)) for k in ("ANISO1", "ANISO2")):
+            raise NotImplementedError("anisotropic workspaces are not supported by this engine")
+        self.purity = tuple(bool(h.get(k, False)) for k in ("PURE_E1", "PURE_B1", "PURE_E2", "PURE_B2"))
         self.lmax = int(h["LMAX"])
         self.lmax_mask = int(h["LMAX_MASK"])
         self.ncls = int(h["NCLS"])
@@ -281,7 +295,7 @@ class NmtWorkspace:
         dev = torch.device("cuda", torch.cuda.current_device())
         if dense is None or dense.shape != (L * self.ncls, L * self.ncls):
             raise ValueError("WSP_PRIMARY does not hold a coupling matrix of the stated size")
-        self.layout = _layout(self.ncls, self.spin1, self.spin2)
+        self.layout = _layout(self.ncls, self.spin1, self.spin2, self.purity)
         # the distinct blocks, read back from their first slot in the dense matrix
         d4 = torch.as_tensor(dense, dtype=torch.float64, device=dev).reshape(L, self.ncls, L, self.ncls)
         self.blocks = {}
@@ -323,9 +337,7 @@ class NmtWorkspace:
         for f in (fl1, fl2):
             if f.spin not in (0, 2):
                 raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
-            if f.pure_e or f.pure_b:
-                raise NotImplementedError("the pure-E/B coupling blocks are not built; purified fields give "
-                                          "alms, maps and coupled spectra only")
+        self.purity = tuple(bool(v) for v in (fl1.pure_e, fl1.pure_b, fl2.pure_e, fl2.pure_b))
         self.spin1, self.spin2 = fl1.spin, fl2.spin
         self.nmaps1 = 1 if self.spin1 == 0 else 2
         self.nmaps2 = 1 if self.spin2 == 0 else 2
@@ -350,11 +362,18 @@ class NmtWorkspace:
             self.wawb = float(torch.sum(fl1.mask * fl2.mask)) * (4 * np.pi / npix) / (4 * np.pi)
         self.toeplitz = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
         self.coupling_info = {"form": "toeplitz" if self.toeplitz else nmt_params.form}
-        if self.toeplitz is None and nmt_params.form == "fast":
+        if any(self.purity):
+            # exact standard blocks (Toeplitz is rejected with purification) and the
+            # quadrature blocks of the purified slots
+            from .purecoupling import pure_coupling_blocks
+            self.blocks = dict(coupling_blocks(self.pcl_mask, self.lmax))
+            self.blocks.update(pure_coupling_blocks(self.pcl_mask, self.lmax))
+            self.coupling_info = {"form": "exact+pure"}
+        elif self.toeplitz is None and nmt_params.form == "fast":
             self.blocks, self.coupling_info = fast_coupling_blocks(self.pcl_mask, self.lmax)
         else:
             self.blocks = coupling_blocks(self.pcl_mask, self.lmax, self.toeplitz)
-        self.layout = _layout(self.ncls, self.spin1, self.spin2)
+        self.layout = _layout(self.ncls, self.spin1, self.spin2, self.purity)
         self._postprocess()
 
     def _postprocess(self):
@@ -464,8 +483,6 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
         raise ValueError("is_teb is only valid for spin1=0 and spin2!=0")
     if pure_any and spin1 != 2 and spin2 != 2:
         raise ValueError("pure_any is only valid for spin 2")
-    if pure_any:
-        raise NotImplementedError("purified coupling blocks are not provided by this engine")
     if spin1 not in (0, 2) or spin2 not in (0, 2):
         raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
     toeplitz_sanity(l_toeplitz, l_exact, dl_band, lmax)
@@ -488,8 +505,16 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
         n = min(len(m), 2 * lmax + 1)
         padded[:n] = m[:n]
         b = coupling_blocks(padded, lmax, toe)
-        per_mask.append({"00": b["00"] / cols, "0s": (b["02"] / cols)[None], "pp": (b["22p"] / cols)[None],
-                         "mm": (b["22m"] / cols)[None]})
+        entry = {"00": b["00"] / cols, "0s": (b["02"] / cols)[None], "pp": (b["22p"] / cols)[None],
+                 "mm": (b["22m"] / cols)[None]}
+        if pure_any:
+            # levels 0, 1, 2 along the leading axis (``gmaster/workspaces.py:1317-1325``)
+            from .purecoupling import pure_coupling_blocks
+            p = pure_coupling_blocks(padded, lmax)
+            entry["0s"] = torch.cat([entry["0s"], (p["02pure"] / cols)[None]])
+            entry["pp"] = torch.cat([entry["pp"], (p["22p1"] / cols)[None], (p["22p2"] / cols)[None]])
+            entry["mm"] = torch.cat([entry["mm"], (p["22m1"] / cols)[None], (p["22m2"] / cols)[None]])
+        per_mask.append(entry)
     out = {}
     for name, present in (("00", has_00), ("0s", has_0s), ("pp", has_ss), ("mm", has_ss)):
         if not present:
@@ -497,6 +522,6 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
             continue
         arrays = [v[name] for v in per_mask]
         out[name] = wrap(arrays[0] if one_dimensional else torch.stack(arrays, dim=-3))
-    out.update(pure_any=False, toeplitz={"l_toeplitz": l_toeplitz, "l_exact": l_exact, "dl_band": dl_band},
+    out.update(pure_any=bool(pure_any), toeplitz={"l_toeplitz": l_toeplitz, "l_exact": l_exact, "dl_band": dl_band},
                spins=(spin1, spin2), lmax=lmax, lmax_mask=lmax_mask)
     return out

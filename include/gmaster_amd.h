@@ -221,6 +221,34 @@ int gm_spin_mul(const double* w, const double* p, int64_t npix, double* out, voi
 int gm_alm_filter(const double* alm, int64_t nin, int64_t alm_stride, const double* cl, int64_t nout,
                   int64_t lmax, double* out, int64_t out_stride, void* stream);
 
+/* ---- pure-E/B coupling blocks ------------------------------------------------------------
+ * The purified spin-2 coupling by Gauss-Legendre quadrature over Wigner d-functions
+ * (gmaster/workspaces.py:928-1052).  Tables are degree-major: row l holds every node. */
+/* Ascending nodes x and weights w of the order-point Gauss-Legendre rule on [-1, 1] (Newton
+ * on P_order from the standard initial guesses).  Replaces _gauss_legendre
+ * (gmaster/workspaces.py:593-606), which reads them from ducc0. */
+int gm_gauss_legendre(int64_t order, double* x, double* w, void* stream);
+/* out[l][k] = lscale[l] d^l_{mn}(beta_k), cos(beta_k) = x[k], l = 0..lmax (zero below
+ * max(|m|, |n|)); lscale may be NULL (1); |m|, |n| <= 8.  Replaces _wigner_d_table
+ * (gmaster/workspaces.py:609-679), same recurrence, sign and normalisation. */
+int gm_wigner_d_table(const double* x, int64_t nnodes, int64_t m, int64_t n, int64_t lmax,
+                      const double* lscale, double* out, void* stream);
+/* c[k] = sum_l coef[l] d^l_{mn}(beta_k) and, when s is not NULL, s[k] = sum_l (-1)^l coef[l]
+ * d^l_{mn}(beta_k), l <= lmax, without a table: the mask correlations of
+ * gmaster/workspaces.py:973-985. */
+int gm_wigner_d_corr(const double* x, int64_t nnodes, int64_t m, int64_t n, int64_t lmax,
+                     const double* coef, double* c, double* s, void* stream);
+/* out[l][k] = node_weight[k] sum_t coef[t] node_factors[t][k] degree_factors[t][l] tables[t][l][k]
+ * for 1..8 terms (host arrays of device pointers; a NULL factor array or entry is 1, a NULL
+ * node_weight is 1): the "left" tables of gmaster/workspaces.py:999-1037 with the quadrature
+ * weight folded in. */
+int gm_table_combine(int64_t nterms, const double* const* tables, const double* const* node_factors,
+                     const double* const* degree_factors, const double* coef, const double* node_weight,
+                     int64_t L, int64_t nnodes, double* out, void* stream);
+/* even = (total + (-1)^(l1+l2) sgnd) / 2, odd = (total - (-1)^(l1+l2) sgnd) / 2 over L x L
+ * (gmaster/workspaces.py:1038-1042). */
+int gm_parity_split(const double* total, const double* sgnd, int64_t L, double* even, double* odd, void* stream);
+
 /* ---- mixed precision: FP32 contraction ---------------------------------------------------
  * Same arguments as the FP64 entries above.  The recurrence, its tables and seeds stay
  * FP64; each step's value is rounded to FP32 once and the contraction over rings and the

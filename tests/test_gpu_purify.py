@@ -118,5 +118,3 @@ def test_pure_deprojection_bias_matches_oracle():
     ref = odp.deprojection_bias(o, o, guess)
     print(f"pure deprojection bias rel={_rel(got, ref):.2e}", flush=True)
     assert _rel(got, ref) < 1e-9
-    with pytest.raises(NotImplementedError):
-        nmt.NmtWorkspace.from_fields(f, f, nmt.NmtBin.from_lmax_linear(LMAX, 4))
