@@ -65,6 +65,10 @@ SIGNATURES = {
     "gm_wigner_d_corr": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p],
     "gm_table_combine": [_i64, _p, _p, _p, _p, _p, _i64, _i64, _p, _p],
     "gm_parity_split": [_p, _p, _i64, _p, _p, _p],
+    "gm_apodize_work_size": [_i64],
+    "gm_mask_apodize": [_p, _i64, _d, _i64, _p, _p, _p],
+    "gm_gaussian_alms": [_p, _i64, _i64, ctypes.c_uint64, _p, _p],
+    "gm_gauss_draws": [ctypes.c_uint64, _i64, _i64, _p, _p],
     "gm_table_size": [_i64],
     "gm_lat_nchunks": [_i64, _i64],
     "gm_lat_nchunks_mixed": [_i64, _i64, _i64],
@@ -73,7 +77,7 @@ SIGNATURES = {
     "gm_abi_version": [],
     "gm_last_error": [],
 }
-_RET_I64 = {"gm_lat_ring_chunks", "gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
+_RET_I64 = {"gm_apodize_work_size", "gm_lat_ring_chunks", "gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
 
 _lib = None
 

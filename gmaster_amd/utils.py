@@ -11,7 +11,7 @@ import math
 import numpy as np
 import torch
 
-from . import sht
+from . import _lib, sht
 from ._array import device_tensor, wrap
 from ._config import (  # noqa: F401  (re-exported)
     NmtParams,
@@ -108,113 +108,109 @@ def alm2map(alm, spin, map_info, alm_info):
 
 
 # ---------------------------------------------------------------------------- inputs
-def _pix2vec(nside):
-    """Unit vectors of every RING pixel (HEALPix closed forms), host FP64."""
-    out = np.empty((12 * nside * nside, 3))
-    for i in range(1, 4 * nside):
-        north = 4 * nside - i if i > 2 * nside else i
-        if north < nside:
-            z = 1.0 - north * north / (3.0 * nside * nside)
-            nphi, phi0 = 4 * north, math.pi / (4 * north)
-            off = 2 * i * (i - 1) if i < nside else 12 * nside * nside - 2 * north * (north + 1)
-        else:
-            z = 2.0 * (2 * nside - north) / (3.0 * nside)
-            nphi = 4 * nside
-            phi0 = math.pi / (4 * nside) if (north - nside) % 2 == 0 else 0.0
-            off = 2 * nside * (nside - 1) + (i - nside) * 4 * nside
-        if i > 2 * nside:
-            z = -z
-        phi = phi0 + 2 * math.pi * np.arange(nphi) / nphi
-        st = math.sqrt(max(0.0, (1 - z) * (1 + z)))
-        out[off:off + nphi, 0] = st * np.cos(phi)
-        out[off:off + nphi, 1] = st * np.sin(phi)
-        out[off:off + nphi, 2] = z
-    return out
-
-
 def mask_apodization(mask_in, aposize, apotype="C1"):
-    """C1 / C2 apodization of a HEALPix mask (``gmaster/utils.py:61-147``).
+    """C1 / C2 apodization of a HEALPix RING mask on the device (``gmaster/utils.py:61-147``).
 
     ``x = sqrt((1 - cos d) / (1 - cos aposize))`` with ``d`` the angle to the nearest
-    masked pixel; C1 multiplies by ``x - sin(2 pi x)/(2 pi)``, C2 by ``(1 - cos pi x)/2``.
+    masked pixel (mask <= 0); C1 multiplies by ``x - sin(2 pi x)/(2 pi)``, C2 by
+    ``(1 - cos pi x)/2``.  The nearest-masked search is ``gm_mask_apodize``'s ring search.
     "Smooth" needs healpy's ``query_disc`` pixel sets and is not provided.
     """
     if apotype not in ("C1", "C2", "Smooth"):
         raise ValueError(f"Apodization type {apotype} unknown. Choose from ['C1', 'C2', 'Smooth']")
-    if apotype == "Smooth":
-        raise NotImplementedError("Smooth apodization is not provided by this engine")
     if aposize < 0:
         raise ValueError("Apodization scale must be a positive number")
-    mask = np.asarray(torch.as_tensor(mask_in).cpu() if isinstance(mask_in, torch.Tensor) else mask_in,
-                      dtype=np.float64)
+    mask = device_tensor(mask_in)
     if mask.ndim != 1:
         raise ValueError("Mask must be a one-dimensional HEALPix map")
     if aposize == 0:
-        return mask.copy()
-    from scipy.spatial import cKDTree
-
-    nside = int(round(math.sqrt(mask.size / 12)))
+        return wrap(mask.clone())
+    if apotype == "Smooth":
+        raise NotImplementedError("Smooth apodization is not provided by this engine")
+    npix = mask.numel()
+    nside = NmtMapInfo(None, (npix,)).nside
     radius = math.radians(aposize)
-    if int(4 * mask.size * (1 - math.cos(1.2 * radius))) < 2:
+    if int(4 * npix * (1 - math.cos(1.2 * radius))) < 2:
         raise ValueError("Your apodization scale is too small for this pixel size")
-    masked = np.flatnonzero(mask <= 0)
-    if not len(masked):
-        return mask.copy()
-    vec = _pix2vec(nside)
-    dist, _ = cKDTree(vec[masked]).query(vec)
-    dm = 0.5 * dist**2
-    thr = 1 - math.cos(radius)
-    aff = (mask > 0) & (dm < thr)
-    x = np.sqrt(np.maximum(dm[aff], 0) / thr)
-    fac = x - np.sin(2 * np.pi * x) / (2 * np.pi) if apotype == "C1" else 0.5 * (1 - np.cos(np.pi * x))
-    out = mask.copy()
-    out[aff] *= fac
+    mask = mask.contiguous()
+    out = torch.empty_like(mask)
+    work = torch.empty(int(_lib.query("gm_apodize_work_size", nside)), dtype=torch.uint8, device=mask.device)
+    _lib.call("gm_mask_apodize", _lib.ptr(mask), nside, radius, 1 if apotype == "C1" else 2, _lib.ptr(out),
+              _lib.ptr(work), _lib.stream_ptr(mask.device))
+    return wrap(out)
+
+
+def _scale_components(alm, fl, lmax):
+    """``alm[c] * fl[c][l]`` per component (``gm_alm_filter`` with a diagonal filter)."""
+    n = alm.shape[0]
+    cl = torch.zeros((n, n, lmax + 1), dtype=torch.float64, device=alm.device)
+    for c in range(n):
+        cl[c, c] = fl[c]
+    out = torch.empty_like(alm)
+    _lib.call("gm_alm_filter", _lib.ptr(alm), n, alm.shape[1], _lib.ptr(cl), n, lmax, _lib.ptr(out), out.shape[1],
+              _lib.stream_ptr(alm.device))
     return out
 
 
 def synfast_spherical(nside, cls, spin_arr, beam=None, seed=-1, wcs=None, lmax=None):
     """Correlated Gaussian full-sky fields (``gmaster/utils.py:823-927``).
 
-    ``cls`` holds the upper triangle of the field-component covariance, row-major over
-    components (T = 1, spin-s = E, B).  Alms are drawn with NumPy (seeded), maps made by
-    this engine's ``alm2map``.  Statistically equivalent to pymaster's, not bit-equal.
+    ``cls`` holds the upper triangle of the component covariance, row-major over components
+    (one per spin-0 field, E and B per spin-s field).  The per-l covariance root comes from
+    an eigendecomposition on the host (``(lmax+1) x ncomp^2`` numbers); the alms are drawn on
+    the device (``gm_gaussian_alms``, Philox4x32-10 keyed by ``seed``), the beams applied per
+    component and the maps made by this engine's ``alm2map``.  Statistically equivalent to
+    the reference's (whose generator is JAX's), reproducible for a given seed.
     """
     if wcs is not None:
         raise NotImplementedError("CAR (WCS) maps are not supported")
-    spin_arr = [int(s) for s in spin_arr]
-    ncomp = sum(1 if s == 0 else 2 for s in spin_arr)
-    cls = np.atleast_2d(np.asarray(cls, dtype=np.float64))
-    if len(cls) != ncomp * (ncomp + 1) // 2:
-        raise ValueError("Wrong number of power spectra for these spins")
-    lmax = 3 * nside - 1 if lmax is None else int(lmax)
+    if int(nside) <= 0:
+        raise ValueError("nside must be positive")
+    spins = np.asarray(spin_arr, dtype=np.int32)
+    if spins.ndim != 1 or np.any(spins < 0):
+        raise ValueError("Spins must be a one-dimensional array of positive values")
+    comps = np.asarray([1 if s == 0 else 2 for s in spins])
+    ncomp = int(comps.sum())
+    spectra = np.asarray(cls, dtype=np.float64)
+    expected = ncomp * (ncomp + 1) // 2
+    if spectra.ndim != 2 or len(spectra) != expected:
+        raise ValueError(f"Must provide all Cls necessary to simulate all fields ({expected})")
+    if ncomp > 8:
+        raise NotImplementedError("at most 8 map components per draw")
+    minfo = NmtMapInfo(None, (12 * int(nside) ** 2,))
+    lmax = minfo.get_lmax() if lmax is None else int(lmax)
+    lmax = min(lmax, spectra.shape[1] - 1)
     L = lmax + 1
     cov = np.zeros((L, ncomp, ncomp))
     k = 0
     for i in range(ncomp):
         for j in range(i, ncomp):
-            cov[:, i, j] = cov[:, j, i] = cls[k, :L]
+            cov[:, i, j] = cov[:, j, i] = spectra[k, :L]
             k += 1
-    if beam is not None:
-        b = np.atleast_2d(beam)
-        comp_beam = np.concatenate([[b[f]] * (1 if s == 0 else 2) for f, s in enumerate(spin_arr)])
-        cov = cov * comp_beam[:, None, :L].T[:, :, None] * comp_beam[None, :, :L].T[:, None, :]
-    # per-l square root (eigendecomposition, PSD clipping)
     w, v = np.linalg.eigh(cov)
-    root = v * np.sqrt(np.clip(w, 0, None))[:, None, :]
+    scale = np.max(np.abs(w), axis=1, keepdims=True)
+    if np.any(w < -1e-12 * np.maximum(scale, 1)):
+        raise ValueError("Input power spectra do not form positive-semidefinite matrices")
+    root = np.ascontiguousarray(v * np.sqrt(np.maximum(w, 0))[:, None, :])
+    if seed < 0:
+        seed = int(np.random.randint(50_000_000))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    root_d = torch.as_tensor(root, device=dev)
     ai = NmtAlmInfo(lmax)
-    ell, m = ai._ell, ai._m
-    rng = np.random.default_rng(None if seed < 0 else seed)
-    re = rng.standard_normal((ai.nelem, ncomp))
-    im = rng.standard_normal((ai.nelem, ncomp))
-    re = np.einsum("aij,aj->ai", root[ell], re)
-    im = np.einsum("aij,aj->ai", root[ell], im)
-    scale = np.where(m == 0, 1.0, 1 / np.sqrt(2))[:, None]
-    alm = (re * scale + 1j * np.where(m[:, None] == 0, 0, im * scale)).T
-    minfo = NmtMapInfo(None, (12 * nside * nside,))
+    alm = torch.empty((ncomp, ai.nelem), dtype=torch.complex128, device=dev)
+    _lib.call("gm_gaussian_alms", _lib.ptr(root_d), lmax, ncomp, int(seed) & 0xFFFFFFFFFFFFFFFF, _lib.ptr(alm),
+              _lib.stream_ptr(dev))
+    if beam is not None:
+        b = np.asarray(beam, dtype=np.float64)
+        if b.ndim != 2 or len(b) != len(spins):
+            raise ValueError("Must provide one beam per field")
+        if b.shape[1] < L:
+            raise ValueError(f"The beam should be provided to ell = {lmax}")
+        per_comp = torch.as_tensor(np.repeat(b[:, :L], comps, axis=0), device=dev)
+        alm = _scale_components(alm, per_comp, lmax)
     maps = []
     c = 0
-    for s in spin_arr:
-        n = 1 if s == 0 else 2
-        maps.append(alm2map(alm[c:c + n], s, minfo, ai))
+    for s, n in zip(spins, comps):
+        maps.append(alm2map(alm[c:c + n], int(s), minfo, ai).as_subclass(torch.Tensor))
         c += n
-    return wrap(torch.cat([mm.as_subclass(torch.Tensor) for mm in maps]))
+    return wrap(torch.cat(maps))

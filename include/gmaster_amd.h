@@ -249,6 +249,26 @@ int gm_table_combine(int64_t nterms, const double* const* tables, const double* 
  * (gmaster/workspaces.py:1038-1042). */
 int gm_parity_split(const double* total, const double* sgnd, int64_t L, double* even, double* odd, void* stream);
 
+/* ---- input side: mask apodization and Gaussian alms ----------------------------------------
+ * Bytes of device workspace gm_mask_apodize needs at this nside (0 for an invalid nside). */
+int64_t gm_apodize_work_size(int64_t nside);
+/* C1 (kind 1) or C2 (kind 2) apodization of a RING-ordered mask over aposize_rad radians:
+ * unmasked pixels within the scale of the nearest masked pixel (mask <= 0) are multiplied by
+ * x - sin(2 pi x)/(2 pi) or (1 - cos(pi x))/2, x = sqrt((1 - cos d) / (1 - cos aposize)).
+ * out may alias mask.  Replaces the cKDTree search of mask_apodization
+ * (gmaster/utils.py:61-147) with a ring search on the device. */
+int gm_mask_apodize(const double* mask, int64_t nside, double aposize_rad, int64_t kind, double* out, void* work,
+                    void* stream);
+/* out (ncomp x nelem complex, healpy packing) = per-l covariance root (lmax+1 x ncomp x ncomp,
+ * row-major) times complex Gaussian draws with unit variance per real part, scaled by
+ * 1/sqrt(2) and with a zero imaginary part at m = 0 (ncomp <= 8).  The draws come from
+ * Philox4x32-10 keyed by seed, one counter per (element, component).  Replaces
+ * _gaussian_alms (gmaster/utils.py:806-820), whose JAX generator is not reproduced. */
+int gm_gaussian_alms(const double* root, int64_t lmax, int64_t ncomp, uint64_t seed, double* out, void* stream);
+/* The raw normal pairs of the same generator: out[2 (e ndraw + d) + {0, 1}] for elements
+ * e < nelem and draws d < ndraw (nelem * ndraw <= 2^31). */
+int gm_gauss_draws(uint64_t seed, int64_t nelem, int64_t ndraw, double* out, void* stream);
+
 /* ---- mixed precision: FP32 contraction ---------------------------------------------------
  * Same arguments as the FP64 entries above.  The recurrence, its tables and seeds stay
  * FP64; each step's value is rounded to FP32 once and the contraction over rings and the
