@@ -15,7 +15,7 @@ sqrt(TT EE), C_BB = 0.01 C_EE; lmax = 3 nside - 1; bins of 16.
 Two forms are timed in the same run: plain FP64 and the chosen form (``--form``, default
 ``fast``): Legendre passes with an FP64 recurrence and FP32 contraction, the Jacobi
 refinement on corrections, and the coupling from the Toeplitz approximation with a wide
-exact region judged by exact spot checks.  ``value`` is the chosen form's time; ``fp64_s``
+exact region checked by exact spot checks.  ``value`` is the chosen form's time; ``fp64_s``
 the plain form's, and ``bandpower_err_vs_fp64`` the chosen form's largest relative
 bandpower error per spectrum.
 

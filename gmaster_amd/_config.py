@@ -47,7 +47,7 @@ def set_form(form, toeplitz=None, tol=None):
     * ``"fp64"`` (default): the exact Wigner-3j coupling of every entry.
     * ``"fast"``: the Toeplitz approximation with a wide exact region (``toeplitz`` =
       ``(l_toeplitz, l_exact, dl_band)`` in physical multipoles, default (2000, 256, 256)),
-      judged by exact spot checks of random non-exact entries: when their largest error in
+      checked by exact spot checks of random non-exact entries: when their largest error in
       correlation units exceeds ``tol`` (default 1e-3), the exact coupling is used instead.
       Below lmax = 2 l_toeplitz the exact coupling is used (it is cheap there).
     """

@@ -16,7 +16,7 @@ Toeplitz parameters are in physical multipoles, so a network trained on masks at
 
 Only the exact entries the fill reads are evaluated (``gm_coupling_pairs``), so the
 learned form costs O(L^2) instead of the exact coupling's O(L^3).  The exact 3j kernel is
-the judge: ``approximate`` evaluates a random sample of non-exact entries exactly
+the referee: ``approximate`` evaluates a random sample of non-exact entries exactly
 (``gm_coupling_pair_values``) and falls back to the full exact coupling when the
 surrogate's error there exceeds the tolerance.
 

@@ -128,7 +128,7 @@ def coupling_blocks(window, lmax, toeplitz=None):
 
 def fast_coupling_blocks(window, lmax, toeplitz=None, tol=None, n_check=4096, seed=0):
     """The ``"fast"`` form: Toeplitz approximation with a wide exact region (only the exact
-    entries it reads are evaluated), judged by exact values of ``n_check`` random non-exact
+    entries it reads are evaluated), checked against exact values of ``n_check`` random non-exact
     entries; falls back to the exact coupling when their largest error in correlation units
     exceeds ``tol``.  Returns ``(blocks, info)``."""
     from .learned import spot_check

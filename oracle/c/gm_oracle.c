@@ -1,6 +1,6 @@
 /*
  * FP64 C/OpenMP restatement of the two dominant stages of the MASTER path -- ORACLE /
- * CPU BASELINE ONLY (test infrastructure; never linked by the product library).
+ * CPU baseline only (test infrastructure; never linked by the product library).
  *
  *  gmo_lat_ana_s   spin-s Legendre analysis over all rings, direct and mirror channels:
  *                  a_D[l] = sum_r lambda_lm(theta_r) G[r][m],

@@ -98,7 +98,7 @@ def test_fast_form_pipeline_matches_fixture():
         workspaces.clear_coupling_cache()
 
 
-def test_fast_coupling_judge_falls_back_when_tolerance_is_tiny():
+def test_fast_coupling_spot_check_falls_back_when_tolerance_is_tiny():
     from gmaster_amd import workload
     from gmaster_amd.learned import mask_window
     from gmaster_amd.workspaces import clear_coupling_cache, coupling_blocks, fast_coupling_blocks
@@ -110,7 +110,7 @@ def test_fast_coupling_judge_falls_back_when_tolerance_is_tiny():
     exact = {k: v.clone() for k, v in coupling_blocks(win, lmax).items()}
     clear_coupling_cache()
     approx, info = fast_coupling_blocks(win, lmax, toeplitz=(200, 32, 32), tol=1.0)
-    print(f"judge info {info}", flush=True)
+    print(f"spot-check info {info}", flush=True)
     assert not info["fallback"] and info["sample_err"] > 0
     assert not torch.equal(approx["00"], exact["00"])
     clear_coupling_cache()
