@@ -19,7 +19,9 @@ from collections import defaultdict
 _KERNELS = ("k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
             "k_coupling", "k_toeplitz", "k_ring_fwd_s0", "k_ring_fwd_s", "k_ring_inv_s0", "k_ring_inv_s", "k_fold_s0", "k_fold_s",
             "k_finish_s0", "k_finish_s", "k_prep_s0", "k_prep_s", "k_lat_tables", "k_cl_partial", "k_cl_final",
-            "k_bin_block", "k_bin_cell")
+            "k_bin_block", "k_bin_cell", "k_gauss_legendre", "k_wigner_d_table", "k_wigner_d_corr",
+            "k_table_combine", "k_parity_split", "k_ring_table", "k_masked_links", "k_apodize", "k_gaussian_alms",
+            "k_spin_mul", "k_alm_filter", "k_dot_partial", "k_dot_final", "k_deproject")
 
 
 def short(name):
