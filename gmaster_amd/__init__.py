@@ -25,7 +25,8 @@ from .utils import (
     mask_apodization,
     synfast_spherical,
 )
-from .workspaces import NmtWorkspace, compute_coupled_cell, get_master_coefficients
+from .workspaces import (NmtWorkspace, compute_coupled_cell, coupling_precision, get_general_coupling_matrix,
+                         get_master_coefficients, set_coupling_precision)
 
 __all__ = [
     "GmArray",
@@ -39,7 +40,10 @@ __all__ = [
     "compute_full_master",
     "deprojection_bias",
     "get_default_params",
+    "coupling_precision",
+    "get_general_coupling_matrix",
     "get_master_coefficients",
+    "set_coupling_precision",
     "map2alm",
     "map2alm_pair",
     "mask_apodization",

@@ -183,8 +183,8 @@ extern "C" int gm_gauss_legendre(int64_t order, double* x, double* w, void* stre
 extern "C" int gm_wigner_d_table(const double* x, int64_t nnodes, int64_t m, int64_t n, int64_t lmax,
                                  const double* lscale, double* out, void* stream) {
     if (!x || !out) return fail(GM_E_ARG, "gm_wigner_d_table: null pointer");
-    if (nnodes < 1 || lmax < 0 || m < -8 || m > 8 || n < -8 || n > 8)
-        return fail(GM_E_ARG, "gm_wigner_d_table: bad sizes / spins (|m|, |n| <= 8)");
+    if (nnodes < 1 || lmax < 0 || m < -16 || m > 16 || n < -16 || n > 16)
+        return fail(GM_E_ARG, "gm_wigner_d_table: bad sizes / spins (|m|, |n| <= 16)");
     hipLaunchKernelGGL(k_wigner_d_table, dim3((unsigned)((nnodes + 127) / 128)), dim3(128), 0, (hipStream_t)stream, x,
                        (int)nnodes, (int)m, (int)n, (int)lmax, lscale, out);
     return check_launch("gm_wigner_d_table");
@@ -193,8 +193,8 @@ extern "C" int gm_wigner_d_table(const double* x, int64_t nnodes, int64_t m, int
 extern "C" int gm_wigner_d_corr(const double* x, int64_t nnodes, int64_t m, int64_t n, int64_t lmax,
                                 const double* coef, double* c, double* s, void* stream) {
     if (!x || !coef || !c) return fail(GM_E_ARG, "gm_wigner_d_corr: null pointer");
-    if (nnodes < 1 || lmax < 0 || m < -8 || m > 8 || n < -8 || n > 8)
-        return fail(GM_E_ARG, "gm_wigner_d_corr: bad sizes / spins (|m|, |n| <= 8)");
+    if (nnodes < 1 || lmax < 0 || m < -16 || m > 16 || n < -16 || n > 16)
+        return fail(GM_E_ARG, "gm_wigner_d_corr: bad sizes / spins (|m|, |n| <= 16)");
     hipLaunchKernelGGL(k_wigner_d_corr, dim3((unsigned)((nnodes + 127) / 128)), dim3(128), 0, (hipStream_t)stream, x,
                        (int)nnodes, (int)m, (int)n, (int)lmax, coef, c, s);
     return check_launch("gm_wigner_d_corr");

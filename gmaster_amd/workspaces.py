@@ -150,6 +150,71 @@ def clear_coupling_cache():
     _COUPLING_CACHE.clear()
 
 
+_COUPLING_PRECISION = ["auto"]
+
+
+def set_coupling_precision(name):
+    """Operand precision of the coupling builders (``gmaster/workspaces.py:56-78``): "fp64",
+    "fp32" or "auto".  Accepted for compatibility; every coupling kernel here evaluates in
+    FP64 (the 3j recursion and the quadrature GEMMs), i.e. at or above the requested width."""
+    if name not in ("fp64", "fp32", "auto"):
+        raise KeyError("GMaster coupling precision must be 'fp64', 'fp32' or 'auto'")
+    _COUPLING_PRECISION[0] = name
+
+
+def coupling_precision():
+    """The current coupling-precision setting ("fp64", "fp32" or "auto")."""
+    return _COUPLING_PRECISION[0]
+
+
+def get_general_coupling_matrix(pcl_mask, s1, s2, n1, n2, parity="all"):
+    """General mode-coupling matrix, as in ``pymaster`` (``gmaster/workspaces.py:1069-1122``):
+
+        M[l, l'] = sum_L (2l' + 1)(2L + 1)/(4 pi) C_L P(l + l' + L)
+                   (l l' L; n1 -s1 s1-n1)(l l' L; n2 -s2 s2-n2)
+
+    over ``0 <= l, l', L <= nl - 1`` (nl = len(pcl_mask)), by Gauss-Legendre quadrature on the
+    device: the product of two 3j symbols is half the integral of d^l_{n1 n2} d^l'_{-s1 -s2}
+    d^L_{s1-n1, s2-n2}, so M = (2l' + 1)/(8 pi) sum_k w_k c(x_k) d^l_{n1 n2}(x_k) d^l'_{-s1 -s2}(x_k)
+    with the correlation c = sum_L (2L + 1) C_L d^L (``gm_wigner_d_corr``; its (-1)^L form
+    gives the parity split), one FP64 GEMM per parity part.  Rows and columns below
+    max(s1, s2, n1, n2) are zero.  ``parity``: "all", "even", "odd" or "both" (even, odd)."""
+    from . import purecoupling as pc
+    if parity not in ("all", "even", "odd", "both"):
+        raise ValueError('`parity` must be "all", "even", "odd", or "both".')
+    w = device_tensor(pcl_mask)
+    if w.ndim != 1:
+        raise ValueError("pcl_mask must be one-dimensional")
+    s1, s2, n1, n2 = (int(v) for v in (s1, s2, n1, n2))
+    for a, b in ((n1, n2), (-s1, -s2), (s1 - n1, s2 - n2)):
+        if max(abs(a), abs(b)) > 16:
+            raise NotImplementedError("spin indices above 16 are not supported")
+    lmax = w.numel() - 1
+    L = lmax + 1
+    dev = w.device
+    # the integrand is a polynomial of degree <= 3 lmax in cos(beta)
+    x, wq = pc.gauss_legendre(3 * lmax // 2 + 2, dev)
+    ell = torch.arange(L, dtype=torch.float64, device=dev)
+    c, sg = pc.wigner_d_corr(x, s1 - n1, s2 - n2, lmax, ((2 * ell + 1) * w).contiguous())
+    left = pc.wigner_d_table(x, n1, n2, lmax)
+    right = pc.wigner_d_table(x, -s1, -s2, lmax, ((2 * ell + 1) / (8 * np.pi)).contiguous())
+    total = torch.matmul(pc.table_combine([(1, left, c, None)], wq), right.T)
+    out = None
+    if parity == "all":
+        out = total
+    else:
+        signed = torch.matmul(pc.table_combine([(1, left, sg, None)], wq), right.T)
+        even, odd = pc.parity_split(total, signed)
+        out = {"even": even, "odd": odd, "both": None}[parity]
+        if parity == "both":
+            out = torch.stack([even, odd])
+    lstart = max(s1, s2, n1, n2)
+    if lstart > 0:
+        out[..., :lstart, :] = 0
+        out[..., :, :lstart] = 0
+    return wrap(out)
+
+
 def compute_coupled_cell(f1, f2):
     """``(n1 n2, lmax + 1)`` coupled pseudo-spectra of two fields (rows ``a * n2 + b``)."""
     if not f1.is_compatible(f2, strict=False):
@@ -304,6 +369,19 @@ class NmtWorkspace:
                 self.blocks[name] = (sign * d4[:, c1, :, c2]).contiguous()
         for name in ("00", "02", "22p", "22m"):
             self.blocks.setdefault(name, torch.zeros((L, L), dtype=torch.float64, device=dev))
+        # a matrix that the slot layout does not reproduce (one set by update_coupling_matrix)
+        # is kept as one block per channel pair
+        again = torch.zeros_like(d4)
+        for name, (c1, c2), sign in self.layout:
+            again[:, c1, :, c2] += sign * self.blocks[name]
+        if not torch.equal(again, d4):
+            self.blocks = {"00": d4[:, 0, :, 0].contiguous()}
+            self.layout = []
+            for c1 in range(self.ncls):
+                for c2 in range(self.ncls):
+                    self.blocks[f"c{c1}_{c2}"] = d4[:, c1, :, c2].contiguous()
+                    self.layout.append((f"c{c1}_{c2}", (c1, c2), 1.0))
+        del again
         _, beams = fits.find(hdus, "BEAMS")
         if "BEAMS" in beams:
             common = np.sqrt(beams["BEAMS"].astype(np.float64))
@@ -453,6 +531,26 @@ class NmtWorkspace:
             sol = torch.linalg.solve(self.mcm_binned, one.reshape(nb * ncls, L * ncls))
             self._bpws = sol.reshape(nb, ncls, L, ncls).permute(1, 0, 3, 2).contiguous()
         return wrap(self._bpws)
+
+    def update_coupling_matrix(self, new_matrix):
+        """Replace the unbinned MCM (``(ncls (lmax+1))^2``, ``l * ncls + c`` ordering) and
+        recompute the binned operators (``gmaster/workspaces.py:2023-2040``)."""
+        size = self.ncls * (self.lmax + 1)
+        expected = (size, size)
+        if tuple(np.shape(new_matrix)) != expected:
+            raise ValueError(f"Input matrix has an inconsistent shape. Expected {expected}, "
+                             f"but got {tuple(np.shape(new_matrix))}")
+        L, ncls = self.lmax + 1, self.ncls
+        dense = device_tensor(new_matrix, device=self.mcm_binned.device).reshape(L, ncls, L, ncls)
+        # every channel pair becomes its own block with sign 1
+        self.blocks = {"00": dense[:, 0, :, 0].contiguous()}
+        self.layout = []
+        for c1 in range(ncls):
+            for c2 in range(ncls):
+                name = f"c{c1}_{c2}"
+                self.blocks[name] = dense[:, c1, :, c2].contiguous()
+                self.layout.append((name, (c1, c2), 1.0))
+        self._postprocess()
 
     def update_beams(self, beam1, beam2):
         L = self.lmax + 1

@@ -229,13 +229,13 @@ int gm_alm_filter(const double* alm, int64_t nin, int64_t alm_stride, const doub
  * (gmaster/workspaces.py:593-606), which reads them from ducc0. */
 int gm_gauss_legendre(int64_t order, double* x, double* w, void* stream);
 /* out[l][k] = lscale[l] d^l_{mn}(beta_k), cos(beta_k) = x[k], l = 0..lmax (zero below
- * max(|m|, |n|)); lscale may be NULL (1); |m|, |n| <= 8.  Replaces _wigner_d_table
+ * max(|m|, |n|)); lscale may be NULL (1); |m|, |n| <= 16.  Replaces _wigner_d_table
  * (gmaster/workspaces.py:609-679), same recurrence, sign and normalisation. */
 int gm_wigner_d_table(const double* x, int64_t nnodes, int64_t m, int64_t n, int64_t lmax,
                       const double* lscale, double* out, void* stream);
 /* c[k] = sum_l coef[l] d^l_{mn}(beta_k) and, when s is not NULL, s[k] = sum_l (-1)^l coef[l]
- * d^l_{mn}(beta_k), l <= lmax, without a table: the mask correlations of
- * gmaster/workspaces.py:973-985. */
+ * d^l_{mn}(beta_k), l <= lmax, without a table (|m|, |n| <= 16): the mask correlations of
+ * gmaster/workspaces.py:973-985 and of the general coupling (:682-732, :1069-1122). */
 int gm_wigner_d_corr(const double* x, int64_t nnodes, int64_t m, int64_t n, int64_t lmax,
                      const double* coef, double* c, double* s, void* stream);
 /* out[l][k] = node_weight[k] sum_t coef[t] node_factors[t][k] degree_factors[t][l] tables[t][l][k]

@@ -129,3 +129,38 @@ def assemble_mcm_teb(blocks, spin2=2):
         out[:, 3 + c, :, 3 + c] += M22p
         out[:, 3 + c, :, 6 - c] += (-1.0 if c in (1, 2) else 1.0) * M22m
     return out.reshape(L * 7, L * 7)
+
+
+def general_coupling_exact(pcl, s1, s2, n1, n2, parity="all"):
+    """``get_general_coupling_matrix`` (``gmaster/workspaces.py:1069-1122``) from exact Racah
+    3j symbols (tiny sizes):
+
+        M[l, l'] = sum_L (2l' + 1)(2L + 1)/(4 pi) C_L P(l + l' + L)
+                   (l l' L; n1 -s1 s1-n1)(l l' L; n2 -s2 s2-n2),
+
+    rows and columns below max(s1, s2, n1, n2) zeroed; ``parity`` "all", "even", "odd" or
+    "both" (even and odd stacked)."""
+    from .wigner3j import threej_exact
+
+    C = np.asarray(pcl, dtype=np.float64)
+    L = len(C)
+    out = np.zeros((2, L, L))
+    for l1 in range(L):
+        for l2 in range(L):
+            for l3 in range(abs(l1 - l2), min(l1 + l2, L - 1) + 1):
+                a = threej_exact(l1, l2, l3, n1, -s1, s1 - n1)
+                if a == 0:
+                    continue
+                b = threej_exact(l1, l2, l3, n2, -s2, s2 - n2)
+                out[(l1 + l2 + l3) % 2, l1, l2] += (2 * l3 + 1) * C[l3] * a * b
+    out *= (2 * np.arange(L) + 1)[None, None, :] / (4 * math.pi)
+    lstart = max(s1, s2, n1, n2)
+    out[:, :lstart, :] = 0
+    out[:, :, :lstart] = 0
+    if parity == "both":
+        return out
+    if parity == "even":
+        return out[0]
+    if parity == "odd":
+        return out[1]
+    return out.sum(axis=0)

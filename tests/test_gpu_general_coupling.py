@@ -1,0 +1,81 @@
+"""GPU parity of ``get_general_coupling_matrix`` (quadrature on the device) against exact
+Racah 3j sums, of its spin-0/2 cases against the 3j coupling kernel at lmax 511, and of
+``NmtWorkspace.update_coupling_matrix`` (binned operators, products, FITS round trip)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.coupling import general_coupling_exact
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("spins", [(0, 0, 0, 0), (0, 2, 0, 2), (2, 2, 2, 2), (3, 2, -1, 1), (1, 3, 1, 3), (4, 0, 2, 0)])
+def test_general_coupling_matches_exact_3j(spins):
+    import gmaster_amd as nmt
+    rng = np.random.default_rng(12)
+    C = rng.uniform(size=8)
+    for parity in ("all", "even", "odd", "both"):
+        got = np.asarray(nmt.get_general_coupling_matrix(C, *spins, parity=parity))
+        want = general_coupling_exact(C, *spins, parity=parity)
+        assert got.shape == want.shape
+        assert np.max(np.abs(got - want)) < 8e-14, (spins, parity)
+    with pytest.raises(ValueError, match="parity"):
+        nmt.get_general_coupling_matrix(C, 0, 0, 0, 0, parity="bad")
+
+
+def test_general_coupling_equals_3j_kernel_blocks_at_lmax_511():
+    import gmaster_amd as nmt
+    from gmaster_amd.workspaces import clear_coupling_cache, coupling_blocks
+    lmax = 511
+    ell = np.arange(lmax + 1)
+    rng = np.random.default_rng(3)
+    C = rng.uniform(0.5, 1, lmax + 1) * 4 * np.pi / (1 + ell / 3.0) ** 3
+    clear_coupling_cache()
+    # the 3j kernel couples window multipoles up to 2 lmax; give it the same window, zero above lmax
+    ref = coupling_blocks(torch.as_tensor(np.concatenate([C, np.zeros(lmax)]), device="cuda"), lmax)
+    cases = ((0, 0, 0, 0, "all", "00"), (0, 2, 0, 2, "all", "02"), (2, 2, 2, 2, "even", "22p"),
+             (2, 2, 2, 2, "odd", "22m"))
+    for s1, s2, n1, n2, parity, name in cases:
+        got = nmt.get_general_coupling_matrix(C, s1, s2, n1, n2, parity=parity).as_subclass(torch.Tensor)
+        want = ref[name]
+        err = (got - want).abs().max().item() / want.abs().max().item()
+        print(f"general ({s1},{s2},{n1},{n2}) {parity} vs {name}: rel {err:.1e}", flush=True)
+        assert err < 1e-11
+
+
+def test_update_coupling_matrix():
+    import os
+    import tempfile
+
+    import gmaster_amd as nmt
+    from oracle.geometry import Rings
+    nside, lmax = 16, 47
+    R = Rings(nside)
+    theta = np.repeat(R.theta, R.nphi)
+    mask = 0.5 * (1 - np.cos(np.pi * np.clip((np.radians(70.0) - theta) / np.radians(30.0), 0, 1)))
+    rng = np.random.default_rng(4)
+    f0 = nmt.NmtField(mask, rng.normal(size=(1, 12 * nside * nside)), n_iter=1)
+    f2 = nmt.NmtField(mask, rng.normal(size=(2, 12 * nside * nside)), n_iter=1)
+    b = nmt.NmtBin.from_lmax_linear(lmax, 4)
+    w = nmt.NmtWorkspace.from_fields(f0, f2, b)
+    dense = w.get_coupling_matrix().cpu().numpy()
+    new = dense + 1e-3 * rng.normal(size=dense.shape)  # off-layout entries too
+    with pytest.raises(ValueError):
+        w.update_coupling_matrix(new[:-1])
+    w.update_coupling_matrix(new)
+    assert np.max(np.abs(w.get_coupling_matrix().cpu().numpy() - new)) == 0
+    cl = rng.uniform(size=(2, lmax + 1))
+    L = lmax + 1
+    coupled = (new @ cl.T.reshape(-1)).reshape(L, 2).T
+    assert np.max(np.abs(w.couple_cell(cl).cpu().numpy() - coupled)) < 1e-13
+    # decoupling inverts the binned matrix built from the new one
+    bp = w.decouple_cell(w.couple_cell(np.ones((2, lmax + 1)))).cpu().numpy()
+    assert np.isfinite(bp).all()
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "w.fits")
+        w.write_to(fn)
+        w2 = nmt.NmtWorkspace.from_file(fn)
+    assert np.max(np.abs(w2.get_coupling_matrix().cpu().numpy() - new)) == 0
+    assert np.max(np.abs(w2.decouple_cell(cl).cpu().numpy() - w.decouple_cell(cl).cpu().numpy())) < 1e-12

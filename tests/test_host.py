@@ -115,3 +115,13 @@ def test_residual_targets_handle_lmax_below_the_grid():
     ok = I <= lmax
     want = 0.5 * S[J[ok], I[ok]] / np.sqrt(D[J[ok]] * D[I[ok]])
     np.testing.assert_allclose(t.numpy()[: len(J)][ok], want, rtol=1e-6)
+
+
+def test_coupling_precision_setting():
+    import gmaster_amd as nmt
+    assert nmt.coupling_precision() == "auto"
+    nmt.set_coupling_precision("fp32")
+    assert nmt.coupling_precision() == "fp32"
+    nmt.set_coupling_precision("auto")
+    with pytest.raises(KeyError):
+        nmt.set_coupling_precision("fp16")

@@ -137,3 +137,21 @@ def test_teb_assembly_is_the_direct_sum_of_the_separate_layouts():
     off[:, 1:3, :, 1:3] = 0
     off[:, 3:, :, 3:] = 0
     assert not off.any()
+
+
+def test_general_coupling_oracle_reduces_to_the_spin_blocks():
+    """The exact general-coupling restatement equals the 00 / 02 / 22+- blocks for those spins
+    (every parity option), and zeroes rows and columns below max(s1, s2, n1, n2)."""
+    from oracle.coupling import general_coupling_exact
+
+    rng = np.random.default_rng(12)
+    C = rng.uniform(size=8)
+    M00, M02, M22p, M22m = coupling.coupling_blocks_exact(np.concatenate([C, np.zeros(7)]), 7)
+    np.testing.assert_allclose(general_coupling_exact(C, 0, 0, 0, 0), M00, atol=1e-15)
+    np.testing.assert_allclose(general_coupling_exact(C, 0, 2, 0, 2), M02, atol=1e-15)
+    both = general_coupling_exact(C, 2, 2, 2, 2, "both")
+    np.testing.assert_allclose(both[0], M22p, atol=1e-15)
+    np.testing.assert_allclose(both[1], M22m, atol=1e-15)
+    np.testing.assert_allclose(general_coupling_exact(C, 2, 2, 2, 2), M22p + M22m, atol=1e-15)
+    g = general_coupling_exact(C, 3, 2, -1, 1)
+    assert np.all(g[:3] == 0) and np.all(g[:, :3] == 0) and np.abs(g).max() > 1e-3
