@@ -167,6 +167,58 @@ def coupling_precision():
     return _COUPLING_PRECISION[0]
 
 
+def _general_blocks(mask_cls, lmax, s1, s2, n1, n2, split=True):
+    """``(even, odd)`` (or their sum when ``split`` is False) ``(lmax+1)^2`` general coupling
+    blocks from the mask spectrum ``mask_cls`` (multipoles L < len(mask_cls); those above
+    2 lmax cannot couple), by Gauss-Legendre quadrature (``gmaster/workspaces.py:682-732,
+    878-905``): ``M = (2l' + 1)/(8 pi) sum_k w_k c(x_k) d^l_{n1 n2}(x_k) d^l'_{-s1 -s2}(x_k)``."""
+    from . import purecoupling as pc
+    for a, b in ((n1, n2), (-s1, -s2), (s1 - n1, s2 - n2)):
+        if max(abs(a), abs(b)) > 16:
+            raise NotImplementedError("spin indices above 16 are not supported")
+    w = mask_cls[: 2 * lmax + 1].contiguous()
+    lw = w.numel() - 1
+    L = lmax + 1
+    dev = w.device
+    # the integrand is a polynomial of degree <= 2 lmax + lw in cos(beta)
+    x, wq = pc.gauss_legendre((2 * lmax + lw) // 2 + 1, dev)
+    ellw = torch.arange(lw + 1, dtype=torch.float64, device=dev)
+    ell = torch.arange(L, dtype=torch.float64, device=dev)
+    c, sg = pc.wigner_d_corr(x, s1 - n1, s2 - n2, lw, ((2 * ellw + 1) * w).contiguous(), signed=split)
+    left = pc.wigner_d_table(x, n1, n2, lmax)
+    right = pc.wigner_d_table(x, -s1, -s2, lmax, ((2 * ell + 1) / (8 * np.pi)).contiguous())
+    total = torch.matmul(pc.table_combine([(1, left, c, None)], wq), right.T)
+    if not split:
+        return total
+    signed = torch.matmul(pc.table_combine([(1, left, sg, None)], wq), right.T)
+    return pc.parity_split(total, signed)
+
+
+def general_spin_blocks(mask_cls, lmax, spin1, spin2, is_teb=False, toeplitz=None):
+    """``{"02", "22p", "22m"}`` blocks of a spin pair outside {0, 2} x {0, 2}
+    (``gmaster/workspaces.py:1801-1827``): the 0-s block sums both parities of
+    (s1, s2, n1, n2) = (spin1, spin2, spin1, spin2); an s x s pair splits them; a joint TEB
+    workspace takes its s x s blocks from (spin2, spin2, spin2, spin2).  With
+    ``toeplitz = (l_toeplitz, l_exact, dl_band)`` each block is then filled by ``gm_toeplitz``."""
+    w = device_tensor(mask_cls)
+    L = lmax + 1
+    out = {}
+    if spin1 == 0 or spin2 == 0:
+        out["02"] = _general_blocks(w, lmax, spin1, spin2, spin1, spin2, split=False)
+    else:
+        out["22p"], out["22m"] = _general_blocks(w, lmax, spin1, spin2, spin1, spin2)
+    if is_teb:
+        out["22p"], out["22m"] = _general_blocks(w, lmax, spin2, spin2, spin2, spin2)
+    if toeplitz is not None and toeplitz[0] > 0:
+        lt, le, dl = (int(v) for v in toeplitz)
+        for k in list(out):
+            filled = torch.empty((L, L), dtype=torch.float64, device=w.device)
+            _lib.call("gm_toeplitz", _lib.ptr(out[k].contiguous()), lmax, lt, le, dl, _lib.ptr(filled),
+                      _lib.stream_ptr(w.device))
+            out[k] = filled
+    return out
+
+
 def get_general_coupling_matrix(pcl_mask, s1, s2, n1, n2, parity="all"):
     """General mode-coupling matrix, as in ``pymaster`` (``gmaster/workspaces.py:1069-1122``):
 
@@ -179,33 +231,18 @@ def get_general_coupling_matrix(pcl_mask, s1, s2, n1, n2, parity="all"):
     with the correlation c = sum_L (2L + 1) C_L d^L (``gm_wigner_d_corr``; its (-1)^L form
     gives the parity split), one FP64 GEMM per parity part.  Rows and columns below
     max(s1, s2, n1, n2) are zero.  ``parity``: "all", "even", "odd" or "both" (even, odd)."""
-    from . import purecoupling as pc
     if parity not in ("all", "even", "odd", "both"):
         raise ValueError('`parity` must be "all", "even", "odd", or "both".')
     w = device_tensor(pcl_mask)
     if w.ndim != 1:
         raise ValueError("pcl_mask must be one-dimensional")
     s1, s2, n1, n2 = (int(v) for v in (s1, s2, n1, n2))
-    for a, b in ((n1, n2), (-s1, -s2), (s1 - n1, s2 - n2)):
-        if max(abs(a), abs(b)) > 16:
-            raise NotImplementedError("spin indices above 16 are not supported")
     lmax = w.numel() - 1
-    L = lmax + 1
-    dev = w.device
-    # the integrand is a polynomial of degree <= 3 lmax in cos(beta)
-    x, wq = pc.gauss_legendre(3 * lmax // 2 + 2, dev)
-    ell = torch.arange(L, dtype=torch.float64, device=dev)
-    c, sg = pc.wigner_d_corr(x, s1 - n1, s2 - n2, lmax, ((2 * ell + 1) * w).contiguous())
-    left = pc.wigner_d_table(x, n1, n2, lmax)
-    right = pc.wigner_d_table(x, -s1, -s2, lmax, ((2 * ell + 1) / (8 * np.pi)).contiguous())
-    total = torch.matmul(pc.table_combine([(1, left, c, None)], wq), right.T)
-    out = None
     if parity == "all":
-        out = total
+        out = _general_blocks(w, lmax, s1, s2, n1, n2, split=False)
     else:
-        signed = torch.matmul(pc.table_combine([(1, left, sg, None)], wq), right.T)
-        even, odd = pc.parity_split(total, signed)
-        out = {"even": even, "odd": odd, "both": None}[parity]
+        even, odd = _general_blocks(w, lmax, s1, s2, n1, n2)
+        out = {"even": even, "odd": odd}.get(parity)
         if parity == "both":
             out = torch.stack([even, odd])
     lstart = max(s1, s2, n1, n2)
@@ -412,9 +449,6 @@ class NmtWorkspace:
         toeplitz_sanity(l_toeplitz, l_exact, dl_band, fl1.ainfo.lmax, (fl1, fl2))
         if normalization not in ("MASTER", "FKP"):
             raise ValueError(f"Unknown normalization type {normalization}. Allowed options are 'MASTER' and 'FKP'")
-        for f in (fl1, fl2):
-            if f.spin not in (0, 2):
-                raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
         self.purity = tuple(bool(v) for v in (fl1.pure_e, fl1.pure_b, fl2.pure_e, fl2.pure_b))
         self.spin1, self.spin2 = fl1.spin, fl2.spin
         self.nmaps1 = 1 if self.spin1 == 0 else 2
@@ -440,7 +474,17 @@ class NmtWorkspace:
             self.wawb = float(torch.sum(fl1.mask * fl2.mask)) * (4 * np.pi / npix) / (4 * np.pi)
         self.toeplitz = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
         self.coupling_info = {"form": "toeplitz" if self.toeplitz else nmt_params.form}
-        if any(self.purity):
+        if self.spin1 not in (0, 2) or self.spin2 not in (0, 2):
+            # general spins: the TT block from the 3j kernel (joint TEB), the others by
+            # quadrature over the mask spectrum up to lmax_mask
+            if self.is_teb:
+                self.blocks = dict(coupling_blocks(self.pcl_mask, self.lmax, self.toeplitz))
+            else:
+                self.blocks = {"00": torch.zeros(1, dtype=torch.float64, device=self.pcl_mask.device)}
+            self.blocks.update(general_spin_blocks(self.pcl_mask, self.lmax, self.spin1, self.spin2, self.is_teb,
+                                                   self.toeplitz))
+            self.coupling_info = {"form": "general-spin quadrature" + (" + toeplitz" if self.toeplitz else "")}
+        elif any(self.purity):
             # exact standard blocks (Toeplitz is rejected with purification) and the
             # quadrature blocks of the purified slots
             from .purecoupling import pure_coupling_blocks
@@ -574,15 +618,16 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
     """Distinct MASTER coupling blocks of a spin pair, divided by ``2 l' + 1``
     (``gmaster/workspaces.py:1197-1350``): ``"00"``, ``"0s"``, ``"pp"`` (M+) and ``"mm"`` (M-),
     None where the spin pair has no such block; ``"0s"``, ``"pp"`` and ``"mm"`` carry a
-    leading axis of length 1 (the purification level), and a mask axis before the last two
-    when ``pcl_mask`` is 2D.  Spins 0 and 2 only."""
+    leading axis of the purification levels (1, or 2 / 3 with ``pure_any``), and a mask axis
+    before the last two when ``pcl_mask`` is 2D.  Spin pairs outside {0, 2} take their 0-s and
+    s x s blocks from the general quadrature (``general_spin_blocks``)."""
     spin1, spin2, lmax = int(spin1), int(spin2), int(lmax)
     if is_teb and not (spin1 == 0 and spin2 != 0):
         raise ValueError("is_teb is only valid for spin1=0 and spin2!=0")
     if pure_any and spin1 != 2 and spin2 != 2:
         raise ValueError("pure_any is only valid for spin 2")
-    if spin1 not in (0, 2) or spin2 not in (0, 2):
-        raise NotImplementedError("coupling matrices are provided for spin 0 and spin 2 fields")
+    if pure_any and (spin1 not in (0, 2) or spin2 not in (0, 2)):
+        raise ValueError("Purification is only implemented for spin 2")
     toeplitz_sanity(l_toeplitz, l_exact, dl_band, lmax)
     masks = device_tensor(pcl_mask)
     one_dimensional = masks.ndim == 1
@@ -594,7 +639,7 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
     dev = masks.device
     cols = (2 * torch.arange(lmax + 1, dtype=torch.float64, device=dev) + 1)[None]
     has_00 = (spin1 == spin2 == 0) or is_teb
-    has_0s = (spin1 != spin2) or is_teb
+    has_0s = ((spin1 != spin2) and (spin1 == 0 or spin2 == 0)) or is_teb
     has_ss = (spin1 != 0 and spin2 != 0) or is_teb
     toe = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
     per_mask = []
@@ -602,7 +647,17 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
         padded = torch.zeros(2 * lmax + 1, dtype=torch.float64, device=dev)
         n = min(len(m), 2 * lmax + 1)
         padded[:n] = m[:n]
-        b = coupling_blocks(padded, lmax, toe)
+        b = dict(coupling_blocks(padded, lmax, toe))
+        if spin1 not in (0, 2) or spin2 not in (0, 2):
+            # the 0-s block pairs the non-zero spin with 0; s x s blocks use (spin1, spin2)
+            # or (spin2, spin2) for a joint TEB pair (``gmaster/workspaces.py:1279-1316``)
+            if has_0s:
+                pair = (0, spin2) if spin1 == 0 else (spin1, 0)
+                b.update(general_spin_blocks(m, lmax, pair[0], pair[1], toeplitz=toe))
+            if has_ss:
+                ss1 = spin2 if is_teb else spin1
+                gb = general_spin_blocks(m, lmax, ss1, spin2, toeplitz=toe)
+                b["22p"], b["22m"] = gb["22p"], gb["22m"]
         entry = {"00": b["00"] / cols, "0s": (b["02"] / cols)[None], "pp": (b["22p"] / cols)[None],
                  "mm": (b["22m"] / cols)[None]}
         if pure_any:

@@ -79,3 +79,62 @@ def test_update_coupling_matrix():
         w2 = nmt.NmtWorkspace.from_file(fn)
     assert np.max(np.abs(w2.get_coupling_matrix().cpu().numpy() - new)) == 0
     assert np.max(np.abs(w2.decouple_cell(cl).cpu().numpy() - w.decouple_cell(cl).cpu().numpy())) < 1e-12
+
+
+@pytest.mark.parametrize("spins,teb", [((1, 1), False), ((0, 1), False), ((1, 3), False), ((0, 3), True), ((0, 1), True)])
+def test_general_spin_workspace_matches_oracle(spins, teb):
+    """Workspaces of spin-1 / spin-3 fields (ncls 2, 4, 7) against the exact-3j oracle blocks in
+    the reference's slot layout."""
+    import gmaster_amd as nmt
+    from oracle.coupling import assemble_mcm, assemble_mcm_teb, coupling_blocks_exact
+    nside, lmax = 8, 15
+    npix = 12 * nside * nside
+    from oracle.geometry import Rings
+    R = Rings(nside)
+    theta = np.repeat(R.theta, R.nphi)
+    mask = 0.5 * (1 - np.cos(np.pi * np.clip((np.radians(80.0) - theta) / np.radians(40.0), 0, 1)))
+    rng = np.random.default_rng(9)
+    fields = [nmt.NmtField(mask, rng.normal(size=(1 if s == 0 else 2, npix)), spin=s, lmax=lmax, lmax_mask=lmax,
+                           n_iter=1) for s in spins]
+    b = nmt.NmtBin.from_lmax_linear(lmax, 4)
+    w = nmt.NmtWorkspace.from_fields(fields[0], fields[1], b, is_teb=teb)
+    pcl = w.pcl_mask.cpu().numpy()
+    s1, s2 = spins
+    M00 = coupling_blocks_exact(pcl, lmax)[0]
+    if s1 == 0:
+        M0s = general_coupling_exact(pcl, 0, s2, 0, s2)
+    else:
+        M0s = np.zeros_like(M00)
+    if teb:
+        Mp, Mm = general_coupling_exact(pcl, s2, s2, s2, s2, "both")
+        ref = assemble_mcm_teb((M00, M0s, Mp, Mm), spin2=s2)
+    else:
+        if s1 != 0:
+            Mp, Mm = general_coupling_exact(pcl, s1, s2, s1, s2, "both")
+        else:
+            Mp = Mm = np.zeros_like(M00)
+        ref, _ = assemble_mcm((M00, M0s, Mp, Mm), s1, s2)
+    got = w.get_coupling_matrix().cpu().numpy()
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print(f"spins {spins} teb {teb}: ncls {w.ncls} rel {err:.1e}", flush=True)
+    assert err < 1e-12
+    cl = rng.uniform(size=(w.ncls, lmax + 1))
+    assert np.isfinite(w.decouple_cell(cl).cpu().numpy()).all()
+
+
+def test_master_coefficients_general_spins():
+    import gmaster_amd as nmt
+    rng = np.random.default_rng(5)
+    lmax = 12
+    pcl = rng.uniform(size=2 * lmax + 1) / (1 + np.arange(2 * lmax + 1)) ** 2
+    cols = 2 * np.arange(lmax + 1) + 1.0
+    out = nmt.get_master_coefficients(pcl, lmax, 1, 3)
+    assert out["00"] is None and out["0s"] is None
+    p, m = general_coupling_exact(pcl, 1, 3, 1, 3, "both")
+    assert np.max(np.abs(np.asarray(out["pp"])[0] - p[: lmax + 1, : lmax + 1] / cols)) < 1e-13
+    assert np.max(np.abs(np.asarray(out["mm"])[0] - m[: lmax + 1, : lmax + 1] / cols)) < 1e-13
+    out = nmt.get_master_coefficients(pcl, lmax, 0, 1, is_teb=True)
+    zs = general_coupling_exact(pcl, 0, 1, 0, 1)
+    assert np.max(np.abs(np.asarray(out["0s"])[0] - zs[: lmax + 1, : lmax + 1] / cols)) < 1e-13
+    p, m = general_coupling_exact(pcl, 1, 1, 1, 1, "both")
+    assert np.max(np.abs(np.asarray(out["pp"])[0] - p[: lmax + 1, : lmax + 1] / cols)) < 1e-13
