@@ -101,14 +101,17 @@ __global__ void __launch_bounds__(kLinkThreads) k_masked_links(const double* __r
     }
 }
 
-__device__ inline double chord2(double x0, double y0, double z0, double x1, double y1, double z1) {
-    const double dx = x0 - x1, dy = y0 - y1, dz = z0 - z1;
-    return 0.5 * (dx * dx + dy * dy + dz * dz);
+// 0.5 |v_p - v_q|^2 between (theta_p, phi_p) and (theta_q, phi_q) without cancellation:
+// 0.5 [(z_p - z_q)^2 + (s_p - s_q)^2] + 2 s_p s_q sin^2((phi_q - phi_p) / 2), da = (phi_q - phi_p) / pi
+__device__ inline double chord2(double base, double ss, double da) {
+    const double h = sinpi(0.5 * da);
+    return fma(2.0 * ss * h, h, base);
 }
 
-// smallest d from the pixel over ring j's candidates, below `best`
+// smallest d from the pixel over ring j's candidates, below `best`; base = the ring's
+// same-phi term, ss = s_p s_j
 __device__ inline double ring_best(const RingT& q, const int* __restrict__ prev, const int* __restrict__ next,
-                                   double ap, double xp, double yp, double zp, double best) {
+                                   double ap, double base, double ss, double best) {
     // pixel of ring q nearest in phi: k = round((phi_p - phi0) nphi / (2 pi)) mod nphi
     double t = (ap - q.a0) * 0.5 * q.nphi;
     int64_t k = (int64_t)floor(t + 0.5);
@@ -117,13 +120,8 @@ __device__ inline double ring_best(const RingT& q, const int* __restrict__ prev,
     const int c0 = prev[q.off + k];
     if (c0 < 0) return best;
     const int c1 = next[q.off + k];
-    double s, c;
-    sincospi(q.a0 + 2.0 * (double)c0 / (double)q.nphi, &s, &c);
-    best = fmin(best, chord2(xp, yp, zp, q.s * c, q.s * s, q.z));
-    if (c1 != c0) {
-        sincospi(q.a0 + 2.0 * (double)c1 / (double)q.nphi, &s, &c);
-        best = fmin(best, chord2(xp, yp, zp, q.s * c, q.s * s, q.z));
-    }
+    best = fmin(best, chord2(base, ss, q.a0 + 2.0 * (double)c0 / (double)q.nphi - ap));
+    if (c1 != c0) best = fmin(best, chord2(base, ss, q.a0 + 2.0 * (double)c1 / (double)q.nphi - ap));
     return best;
 }
 
@@ -140,21 +138,20 @@ __global__ void k_apodize(const double* __restrict__ mask, const RingT* __restri
             continue;
         }
         const double ap = g.a0 + 2.0 * (double)k / (double)g.nphi;
-        double sp, cp;
-        sincospi(ap, &sp, &cp);
-        const double xp = g.s * cp, yp = g.s * sp, zp = g.z;
         double best = thr;
         for (int j = r; j >= 0; --j) {
             const RingT q = rings[j];
             const double ds = q.s - g.s, dz = q.z - g.z;
-            if (0.5 * (ds * ds + dz * dz) >= best) break;
-            best = ring_best(q, prev, next, ap, xp, yp, zp, best);
+            const double base = 0.5 * (ds * ds + dz * dz);
+            if (base >= best) break;
+            best = ring_best(q, prev, next, ap, base, q.s * g.s, best);
         }
         for (int j = r + 1; j < nring; ++j) {
             const RingT q = rings[j];
             const double ds = q.s - g.s, dz = q.z - g.z;
-            if (0.5 * (ds * ds + dz * dz) >= best) break;
-            best = ring_best(q, prev, next, ap, xp, yp, zp, best);
+            const double base = 0.5 * (ds * ds + dz * dz);
+            if (base >= best) break;
+            best = ring_best(q, prev, next, ap, base, q.s * g.s, best);
         }
         double o = mv;
         if (best < thr) {
