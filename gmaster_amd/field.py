@@ -25,9 +25,9 @@ from .utils import NmtAlmInfo, NmtMapInfo
 _MASK_ALMS = {}
 
 
-def _mask_key(mask, lmax_mask, n_iter_mask):
+def _mask_key(mask, lmax_mask, n_iter_mask, precision=None):
     return (mask.data_ptr(), mask._version, mask.numel(), str(mask.device), int(lmax_mask), int(n_iter_mask),
-            _precision())
+            precision or _precision())
 
 
 def _precision():
@@ -108,6 +108,10 @@ class NmtField:
         self.spin = int(spin)
         self.nmaps = 1 if self.spin == 0 else 2
         pure_any = self.pure_e or self.pure_b
+        # Purified fields keep FP64 mask alms in every form: their coupling weights the mask
+        # spectrum by up to (L+2)(L+1)L(L-1) ~ L^4, which turns FP32-level rounding of the mask
+        # alms into ~1e-4 changes of the low-l pure blocks.
+        self._mask_precision = "fp64" if pure_any else _precision()
         if pure_any and self.spin != 2:
             raise ValueError("Purification is only defined for spin-2 fields")
         if pure_any and self.ainfo != self.ainfo_mask:
@@ -144,7 +148,7 @@ class NmtField:
                 alpha_d = torch.as_tensor(self.alphas, dtype=torch.float64, device=dev)
                 _lib.call("gm_map_deproject", _lib.ptr(unmasked), unmasked.numel(), _lib.ptr(temps_unmasked),
                           len(temps), unmasked.numel(), _lib.ptr(alpha_d), _lib.stream_ptr(dev))
-        key = _mask_key(self.mask, lmax_mask, self.n_iter_mask)
+        key = _mask_key(self.mask, lmax_mask, self.n_iter_mask, self._mask_precision)
         self.alm_mask = _MASK_ALMS.get(key)
         if pure_any:
             amask = self.get_mask_alms().as_subclass(torch.Tensor)
@@ -200,13 +204,19 @@ class NmtField:
 
     def get_mask_alms(self):
         if self.alm_mask is None:
-            key = _mask_key(self.mask, self.ainfo_mask.lmax, self.n_iter_mask)
-            alm = _MASK_ALMS.get(key)
-            if alm is None:
-                alm = sht.map2alm_s0(self.mask[None, :], self.ainfo_mask.lmax, n_iter=self.n_iter_mask,
-                                      precision=_precision())[0]
-                _remember(key, alm)
+            alm = self.mask_alms_at(getattr(self, "_mask_precision", _precision()))
             if self.lite:
                 return wrap(alm)
             self.alm_mask = alm
         return wrap(self.alm_mask)
+
+    def mask_alms_at(self, precision):
+        """Mask alms computed with the given transform precision ("fp64" or "mixed"),
+        shared through the mask-alms cache."""
+        key = _mask_key(self.mask, self.ainfo_mask.lmax, self.n_iter_mask, precision)
+        alm = _MASK_ALMS.get(key)
+        if alm is None:
+            alm = sht.map2alm_s0(self.mask[None, :], self.ainfo_mask.lmax, n_iter=self.n_iter_mask,
+                                 precision=precision)[0]
+            _remember(key, alm)
+        return alm

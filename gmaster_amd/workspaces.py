@@ -463,8 +463,13 @@ class NmtWorkspace:
         self.nbands = bins.get_n_bands()
         self.normalization = normalization
         self.norm_type = int(normalization == "FKP")
-        am1 = fl1.get_mask_alms()
-        am2 = am1 if fl2 is fl1 else fl2.get_mask_alms()
+        if any(self.purity):
+            # the pure blocks amplify mask-spectrum rounding (weights up to ~L^4): FP64 mask alms
+            am1 = fl1.mask_alms_at("fp64")
+            am2 = am1 if fl2 is fl1 else fl2.mask_alms_at("fp64")
+        else:
+            am1 = fl1.get_mask_alms()
+            am2 = am1 if fl2 is fl1 else fl2.get_mask_alms()
         self.pcl_mask = _coupled_cell(am1, am2, self.lmax_mask)[0]
         if fl2 is fl1:
             self.pcl_mask = self.pcl_mask - fl1.Nw
