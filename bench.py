@@ -161,6 +161,57 @@ def sht_batch(args):
     }))
 
 
+def sims(args):
+    """Simulations sharing one workspace trio (00, 02, 22) per Nside: per-simulation T+P
+    fields, coupled spectra and decoupling, timed in plain FP64 and in the chosen form; the
+    workspaces are built once per form outside the timed region."""
+    import torch
+
+    import gmaster_amd as nmt
+    from gmaster_amd import field, sht, workload, workspaces
+
+    nside, n = args.nside, args.sims
+    dev = torch.device("cuda", 0)
+    mask = workload.hole_mask(nside, device=dev)
+    maps = [workload.tqu_maps(nside, seed=1000 + s) for s in range(n)]
+    b = nmt.NmtBin.from_nside_linear(nside, args.nlb)
+
+    def fields(tqu):
+        return nmt.NmtField(mask, tqu[0:1], n_iter=3), nmt.NmtField(mask, tqu[1:3], n_iter=3)
+
+    def one(ws, tqu):
+        fT, fP = fields(tqu)
+        return [w.decouple_cell(nmt.compute_coupled_cell(f1, f2)).as_subclass(torch.Tensor)
+                for w, (f1, f2) in zip(ws, ((fT, fT), (fT, fP), (fP, fP)))]
+
+    out = {}
+    for form in ("fp64", "fast"):
+        nmt.set_form(form)
+        sht.clear_plans()
+        workspaces.clear_coupling_cache()
+        field.clear_mask_cache()
+        fT, fP = fields(maps[0])
+        ws = [nmt.NmtWorkspace.from_fields(f1, f2, b) for f1, f2 in ((fT, fT), (fT, fP), (fP, fP))]
+        one(ws, maps[0])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        res = [one(ws, tqu) for tqu in maps]
+        torch.cuda.synchronize()
+        out[form] = ((time.perf_counter() - t) / n, res)
+        log(f"{form}: {out[form][0]:.3f} s per simulation over {n}")
+    nmt.set_form("fp64")
+    worst = max(max(bandpower_errors(g, r).values()) for g, r in zip(out["fast"][1], out["fp64"][1]))
+    print(json.dumps({
+        "metric": f"T+P MASTER simulations/s sharing one workspace trio, HEALPix Nside {nside}",
+        "value": 1.0 / out["fast"][0], "unit": "sims/s", "higher_is_better": True, "n_gpus": 1,
+        "steps": n, "warmup": 1, "dtype": "f64 recurrence, f32 contraction", "data": "synthetic",
+        "config": {"workload": f"{n} Gaussian T+P skies x 20-hole mask, Nside {nside}, lmax {3 * nside - 1}, "
+                               f"bins of {args.nlb}"},
+        "fp64_sims_per_s": 1.0 / out["fp64"][0], "speedup_vs_fp64": out["fp64"][0] / out["fast"][0],
+        "max_bandpower_err_vs_fp64": worst,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +228,8 @@ def main():
     ap.add_argument("--sht-batch", type=int, default=0,
                     help="instead of the MASTER pipeline, time K spin-0 maps through the batched transforms")
     ap.add_argument("--max-group", type=int, default=2, help="maps per shared recurrence for --sht-batch")
+    ap.add_argument("--sims", type=int, default=0,
+                    help="time N T+P simulations sharing one workspace trio (FP64 vs chosen form) instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -185,6 +238,8 @@ def main():
 
     if args.sht_batch > 0:
         return sht_batch(args)
+    if args.sims > 0:
+        return sims(args)
 
     import torch
 

@@ -948,13 +948,19 @@ constexpr int large_r(bool syn, bool mixed, bool spin0, int K = 1) {
                  : (mixed ? (spin0 ? GM_MIX_ANA_R_S0 : GM_MIX_ANA_R_S) : (spin0 ? GM_ANA_R_S0 : GM_ANA_R_S));
 }
 
+// Ring counts up to GM_MID_RINGS use 2-wave workgroups of 4 rings per lane: more, shorter
+// workgroups for the smaller geometries.
+#ifndef GM_MID_RINGS
+#define GM_MID_RINGS 512
+#endif
+
 LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn, int K = 1) {
     const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
     LatConfig cfg;
     // the launchers select the instantiation by NW: NW 1 -> R 2, NW 2 -> R 4, NW 4 -> the
     // large R; CH = 64 NW R must match the kernel's own chunk size
     if (nrings <= 128) cfg = {2, 1, 128, 0, 0};
-    else if (nrings <= 512) cfg = {4, 2, 512, 0, 0};
+    else if (nrings <= GM_MID_RINGS) cfg = {4, 2, 512, 0, 0};
     else {
         const int r = large_r(syn, mixed, spin == 0, K);
         cfg = {r, 4, 256 * r, 0, 0};
@@ -1068,6 +1074,12 @@ int launch_ana_s0(const char* name, const double* spec, int64_t nmaps, int64_t l
     return check_launch(name);
 }
 
+// A single mixed spin-0 map (K = 1) synthesises on the matrix unit when this is 1, else on
+// the vector ALU in FP32.
+#ifndef GM_MIX_SYN0_SINGLE_MFMA
+#define GM_MIX_SYN0_SINGLE_MFMA 1
+#endif
+
 template <typename F>
 int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t nside, int64_t lmax, const double* alpha,
                   const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, double* spec, int64_t ld,
@@ -1089,7 +1101,7 @@ int launch_syn_s0(const char* name, const double* coef, int64_t nmaps, int64_t n
     if (int e = check_config(name, cfg, large_r(true, mixed, true, (int)nmaps))) return e;
 #define GM_SYN0(KK, RR, WW)                                                                                       \
     do {                                                                                                          \
-        if constexpr (mixed)                                                                                      \
+        if constexpr (mixed && (KK > 1 || GM_MIX_SYN0_SINGLE_MFMA))                                               \
             hipLaunchKernelGGL((k_syn_s0_mx<KK, RR, WW>), grid, block, 0, st, cf, (int)nside, (int)L, alpha,      \
                                seed_mant, seed_exp, nactive, sp, ld, smap);                                        \
         else                                                                                                      \

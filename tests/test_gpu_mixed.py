@@ -40,6 +40,17 @@ def test_mixed_kernels_match_fp64_kernels_to_fp32_precision(nside, spin):
         s64, s32 = torch.empty_like(spec), torch.empty_like(spec)
         sht._synthesis_s0(p, a64, s64)
         sht._synthesis_s0(p, a64, s32, mixed=True)
+        # the single-map mixed entry (the package synthesises single maps in FP64)
+        from gmaster_amd import _lib
+        st = _lib.stream_ptr()
+        coef = torch.empty(p.tsize, dtype=torch.complex128, device="cuda")
+        _lib.call("gm_lat_prep_s0", _lib.ptr(a64[0:1]), 1, a64.stride(0), _lib.ptr(p.P), p.lmax, _lib.ptr(coef), st)
+        one = torch.empty_like(spec[0:1])
+        _lib.call("gm_lat_syn_s0_mixed", _lib.ptr(coef), 1, p.nside, p.lmax, _lib.ptr(p.alpha), _lib.ptr(p.smant),
+                  _lib.ptr(p.sexp), _lib.ptr(p.nact), _lib.ptr(one), one.shape[2], one.stride(0), st)
+        e1 = _rel(one.cpu(), s64[0:1].cpu())
+        print(f"  single-map mixed synthesis rel {e1:.2e}", flush=True)
+        assert e1 < 1e-5
     else:
         W = 2 * L - 1
         spec = torch.as_tensor(rng.standard_normal((nring, W)) + 1j * rng.standard_normal((nring, W)), device="cuda")
