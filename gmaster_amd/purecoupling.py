@@ -36,8 +36,18 @@ def gauss_legendre(order, device):
     return x, w * (2.0 / w.sum())
 
 
+def _need(t, n, what):
+    """``t`` is a contiguous FP64 device tensor with at least ``n`` elements (the kernels index
+    it up to n - 1)."""
+    if t.dtype != F64 or not t.is_cuda or not t.is_contiguous() or t.numel() < n:
+        raise ValueError(f"{what}: expected a contiguous float64 device tensor of at least {n} elements")
+
+
 def wigner_d_table(x, m, n, lmax, lscale=None):
     """``(lmax + 1, len(x))`` table ``lscale[l] d^l_{mn}(beta_k)``, cos(beta_k) = x[k]."""
+    _need(x, 1, "nodes")
+    if lscale is not None:
+        _need(lscale, lmax + 1, "lscale")
     out = torch.empty((lmax + 1, x.numel()), dtype=F64, device=x.device)
     _lib.call("gm_wigner_d_table", _lib.ptr(x), x.numel(), int(m), int(n), int(lmax), _lib.ptr(lscale), _lib.ptr(out),
               _lib.stream_ptr(x.device))
@@ -46,9 +56,11 @@ def wigner_d_table(x, m, n, lmax, lscale=None):
 
 def wigner_d_corr(x, m, n, lmax, coef, signed=True):
     """``(c, s)``: ``sum_l coef[l] d^l_{mn}`` and the same with ``(-1)^l`` (None unless ``signed``)."""
+    _need(x, 1, "nodes")
+    coef = coef.contiguous()
+    _need(coef, lmax + 1, "coef")
     c = torch.empty_like(x)
     s = torch.empty_like(x) if signed else None
-    coef = coef.contiguous()
     _lib.call("gm_wigner_d_corr", _lib.ptr(x), x.numel(), int(m), int(n), int(lmax), _lib.ptr(coef), _lib.ptr(c),
               _lib.ptr(s), _lib.stream_ptr(x.device))
     return c, s
@@ -59,6 +71,16 @@ def table_combine(terms, node_weight):
     ``(coef, table, node_factor or None, degree_factor or None)``."""
     n = len(terms)
     L, nnodes = terms[0][1].shape
+    for _, table, u, v in terms:
+        if tuple(table.shape) != (L, nnodes):
+            raise ValueError("table_combine: tables of different shapes")
+        _need(table, L * nnodes, "table")
+        if u is not None:
+            _need(u, nnodes, "node factor")
+        if v is not None:
+            _need(v, L, "degree factor")
+    if node_weight is not None:
+        _need(node_weight, nnodes, "node weight")
     arr = lambda vals: (ctypes.c_void_p * n)(*[None if v is None else v.data_ptr() for v in vals])
     tables = arr([t[1] for t in terms])
     nodes = arr([t[2] for t in terms])
@@ -73,6 +95,9 @@ def table_combine(terms, node_weight):
 
 def parity_split(total, sgnd):
     L = total.shape[0]
+    if tuple(total.shape) != (L, L) or tuple(sgnd.shape) != (L, L):
+        raise ValueError("parity_split: expected two square matrices of the same size")
+    total, sgnd = total.contiguous(), sgnd.contiguous()
     even = torch.empty_like(total)
     odd = torch.empty_like(total)
     _lib.call("gm_parity_split", _lib.ptr(total), _lib.ptr(sgnd), L, _lib.ptr(even), _lib.ptr(odd),
