@@ -13,7 +13,8 @@
 //     distance (the chord to the same phi on ring j) stays below the best so far and the
 //     threshold.
 // The result equals the exhaustive search for every pixel with d below the threshold, the
-// only ones the apodization changes.
+// only ones the apodization changes.  Kind 3 instead zeroes every pixel within the scale of
+// a masked pixel (masked pixels included): the first step of the "Smooth" apodization.
 #include <math.h>
 
 #include "common.h"
@@ -134,7 +135,7 @@ __global__ void k_apodize(const double* __restrict__ mask, const RingT* __restri
         const int64_t p = g.off + k;
         const double mv = mask[p];
         if (!(mv > 0.0)) {
-            out[p] = mv;
+            out[p] = kind == 3 ? 0.0 : mv;
             continue;
         }
         const double ap = g.a0 + 2.0 * (double)k / (double)g.nphi;
@@ -154,7 +155,9 @@ __global__ void k_apodize(const double* __restrict__ mask, const RingT* __restri
             best = ring_best(q, prev, next, ap, base, q.s * g.s, best);
         }
         double o = mv;
-        if (best < thr) {
+        if (best < thr && kind == 3) {
+            o = 0.0;
+        } else if (best < thr) {
             const double x = sqrt(fmax(best, 0.0) / thr);
             double f;
             if (kind == 1) {
@@ -186,7 +189,7 @@ extern "C" int gm_mask_apodize(const double* mask, int64_t nside, double aposize
     if (!mask || !out || !work) return fail(GM_E_ARG, "gm_mask_apodize: null pointer");
     if (nside < 1 || nside > 8192 || (nside & (nside - 1)))
         return fail(GM_E_ARG, "gm_mask_apodize: nside must be a power of 2 <= 8192");
-    if (kind != 1 && kind != 2) return fail(GM_E_ARG, "gm_mask_apodize: kind must be 1 (C1) or 2 (C2)");
+    if (kind < 1 || kind > 3) return fail(GM_E_ARG, "gm_mask_apodize: kind must be 1 (C1), 2 (C2) or 3 (zero)");
     if (!(aposize_rad > 0.0) || aposize_rad > M_PI) return fail(GM_E_ARG, "gm_mask_apodize: aposize out of (0, pi]");
     const int64_t npix = 12 * nside * nside, nring = 4 * nside - 1;
     char* w = (char*)work;

@@ -64,6 +64,7 @@ class NmtField:
         self.temp = self.alm_temp = self.alphas = self.iM = None
         self.Nw = 0.0
         self.Nf = 0.0
+        self.alpha = None  # data-to-randoms ratio of clustering catalogues; None for maps
 
         mask_t = device_tensor(mask)
         if mask_t.ndim != 1:

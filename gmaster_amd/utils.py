@@ -114,7 +114,9 @@ def mask_apodization(mask_in, aposize, apotype="C1"):
     ``x = sqrt((1 - cos d) / (1 - cos aposize))`` with ``d`` the angle to the nearest
     masked pixel (mask <= 0); C1 multiplies by ``x - sin(2 pi x)/(2 pi)``, C2 by
     ``(1 - cos pi x)/2``.  The nearest-masked search is ``gm_mask_apodize``'s ring search.
-    "Smooth" needs healpy's ``query_disc`` pixel sets and is not provided.
+    "Smooth" zeroes the pixels within 2.5 aposize (plus the largest pixel radius, healpy's
+    inclusive disc rule) of a masked pixel with the same search, then smooths through the
+    engine's FP64 transforms.
     """
     if apotype not in ("C1", "C2", "Smooth"):
         raise ValueError(f"Apodization type {apotype} unknown. Choose from ['C1', 'C2', 'Smooth']")
@@ -125,19 +127,44 @@ def mask_apodization(mask_in, aposize, apotype="C1"):
         raise ValueError("Mask must be a one-dimensional HEALPix map")
     if aposize == 0:
         return wrap(mask.clone())
-    if apotype == "Smooth":
-        raise NotImplementedError("Smooth apodization is not provided by this engine")
     npix = mask.numel()
     nside = NmtMapInfo(None, (npix,)).nside
     radius = math.radians(aposize)
-    if int(4 * npix * (1 - math.cos(1.2 * radius))) < 2:
+    scale = 2.5 if apotype == "Smooth" else 1.2
+    if int(4 * npix * (1 - math.cos(scale * radius))) < 2:
         raise ValueError("Your apodization scale is too small for this pixel size")
     mask = mask.contiguous()
     out = torch.empty_like(mask)
     work = torch.empty(int(_lib.query("gm_apodize_work_size", nside)), dtype=torch.uint8, device=mask.device)
-    _lib.call("gm_mask_apodize", _lib.ptr(mask), nside, radius, 1 if apotype == "C1" else 2, _lib.ptr(out),
-              _lib.ptr(work), _lib.stream_ptr(mask.device))
-    return wrap(out)
+    if apotype != "Smooth":
+        _lib.call("gm_mask_apodize", _lib.ptr(mask), nside, radius, 1 if apotype == "C1" else 2, _lib.ptr(out),
+                  _lib.ptr(work), _lib.stream_ptr(mask.device))
+        return wrap(out)
+    # "Smooth": zero every pixel within 2.5 aposize of a masked pixel, widened by the largest
+    # pixel radius as healpy's inclusive query_disc does, then smooth with a Gaussian beam of
+    # width aposize and multiply by the mask (gmaster/utils.py:119-134)
+    if not bool((mask <= 0).any()):
+        return wrap(mask.clone())
+    rbig = min(2.5 * radius + _max_pixrad(nside), math.pi)
+    _lib.call("gm_mask_apodize", _lib.ptr(mask), nside, rbig, 3, _lib.ptr(out), _lib.ptr(work),
+              _lib.stream_ptr(mask.device))
+    lmax = 3 * nside - 1
+    alm = sht.map2alm_s0(out[None, :], lmax, n_iter=3, precision="fp64")
+    ell = torch.arange(lmax + 1, dtype=torch.float64, device=mask.device)
+    alm = _scale_components(alm, torch.exp(-0.5 * ell * (ell + 1) * radius * radius)[None], lmax)
+    return wrap(sht.alm2map_s0(alm, nside, lmax)[0] * mask)
+
+
+def _max_pixrad(nside):
+    """Largest angle between a RING pixel's centre and a point of the pixel: from the centre
+    (z = 2/3, phi = pi / (4 nside)) of the first belt ring's first pixel to its corner at
+    (z = 1 - (1 - 1/nside)^2 / 3, phi = 0)."""
+    za, pa = 2.0 / 3.0, math.pi / (4 * nside)
+    zb = 1.0 - (1.0 - 1.0 / nside) ** 2 / 3.0
+    va = (math.sqrt(1 - za * za) * math.cos(pa), math.sqrt(1 - za * za) * math.sin(pa), za)
+    vb = (math.sqrt(1 - zb * zb), 0.0, zb)
+    cross = (va[1] * vb[2] - va[2] * vb[1], va[2] * vb[0] - va[0] * vb[2], va[0] * vb[1] - va[1] * vb[0])
+    return math.atan2(math.sqrt(sum(c * c for c in cross)), sum(a * b for a, b in zip(va, vb)))
 
 
 def _scale_components(alm, fl, lmax):

@@ -254,8 +254,9 @@ int gm_parity_split(const double* total, const double* sgnd, int64_t L, double* 
 int64_t gm_apodize_work_size(int64_t nside);
 /* C1 (kind 1) or C2 (kind 2) apodization of a RING-ordered mask over aposize_rad radians:
  * unmasked pixels within the scale of the nearest masked pixel (mask <= 0) are multiplied by
- * x - sin(2 pi x)/(2 pi) or (1 - cos(pi x))/2, x = sqrt((1 - cos d) / (1 - cos aposize)).
- * out may alias mask.  Replaces the cKDTree search of mask_apodization
+ * x - sin(2 pi x)/(2 pi) or (1 - cos(pi x))/2, x = sqrt((1 - cos d) / (1 - cos aposize));
+ * kind 3 zeroes every pixel (masked ones included) whose centre lies closer than aposize_rad
+ * to a masked pixel's centre, the other pixels keep their value.  out may alias mask.  Replaces the cKDTree search of mask_apodization
  * (gmaster/utils.py:61-147) with a ring search on the device. */
 int gm_mask_apodize(const double* mask, int64_t nside, double aposize_rad, int64_t kind, double* out, void* work,
                     void* stream);

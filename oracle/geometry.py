@@ -100,6 +100,46 @@ def pix2ang_ring(nside, pix):
     return theta, phi
 
 
+def ang2pix_ring(nside, theta, phi):
+    """RING pixel containing (theta, phi) (Gorski et al. 2005, the cap / belt formulae)."""
+    z = np.cos(np.asarray(theta, dtype=np.float64))
+    za = np.abs(z)
+    tt = np.mod(np.asarray(phi, dtype=np.float64), 2 * np.pi) / (0.5 * np.pi)  # in [0, 4)
+    npix = 12 * nside * nside
+    ncap = 2 * nside * (nside - 1)
+    pix = np.empty(z.shape, dtype=np.int64)
+    belt = za <= 2.0 / 3.0
+    t1 = nside * (0.5 + tt[belt])
+    t2 = nside * z[belt] * 0.75
+    jp = np.floor(t1 - t2).astype(np.int64)  # index of the ascending edge line
+    jm = np.floor(t1 + t2).astype(np.int64)  # index of the descending edge line
+    ir = nside + 1 + jp - jm  # ring index counted from z = 2/3, in 1 .. 2 nside + 1
+    kshift = 1 - (ir & 1)
+    ip = np.mod((jp + jm - nside + kshift + 1) // 2, 4 * nside)
+    pix[belt] = ncap + (ir - 1) * 4 * nside + ip
+    cap = ~belt
+    tc = tt[cap]
+    tp = tc - np.floor(tc)
+    tmp = nside * np.sqrt(3.0 * (1.0 - za[cap]))
+    jp = np.floor(tp * tmp).astype(np.int64)
+    jm = np.floor((1.0 - tp) * tmp).astype(np.int64)
+    ir = jp + jm + 1  # ring index counted from the nearer pole
+    ip = np.mod(np.floor(tc * ir).astype(np.int64), 4 * ir)
+    pix[cap] = np.where(z[cap] > 0, 2 * ir * (ir - 1) + ip, npix - 2 * ir * (ir + 1) + ip)
+    return pix
+
+
+def max_pixrad(nside):
+    """Largest angle between a RING pixel centre and a point of that pixel: from the centre
+    (z = 2/3, phi = pi / (4 nside)) of the first belt ring's first pixel to its corner at
+    (z = 1 - (1 - 1/nside)^2 / 3, phi = 0)."""
+    za, pa = 2.0 / 3.0, np.pi / (4 * nside)
+    zb, pb = 1.0 - (1.0 - 1.0 / nside) ** 2 / 3.0, 0.0
+    va = np.array([np.sqrt(1 - za * za) * np.cos(pa), np.sqrt(1 - za * za) * np.sin(pa), za])
+    vb = np.array([np.sqrt(1 - zb * zb) * np.cos(pb), np.sqrt(1 - zb * zb) * np.sin(pb), zb])
+    return float(np.arctan2(np.linalg.norm(np.cross(va, vb)), va @ vb))
+
+
 def pix2vec_ring(nside, pix):
     theta, phi = pix2ang_ring(nside, pix)
     st = np.sin(theta)

@@ -143,3 +143,16 @@ def test_synfast_matches_oracle_alms_and_maps():
     bad[1] = 10 * np.sqrt(tt * ee)  # |TE| > sqrt(TT EE)
     with pytest.raises(ValueError):
         nmt.synfast_spherical(nside, bad, [0, 2])
+
+
+@pytest.mark.parametrize("nside,apo", [(16, 4.0), (32, 3.0), (64, 1.5)])
+def test_smooth_apodization_matches_oracle(nside, apo):
+    import gmaster_amd as nmt
+    mask = (_holes_mask(nside, nside, nholes=6, weights=False) > 0).astype(np.float64)
+    got = np.asarray(nmt.mask_apodization(mask, apo, "Smooth"))
+    want = oap.smooth_apodization(mask, apo)
+    err = np.max(np.abs(got - want))
+    print(f"Smooth nside {nside} apo {apo}: max err {err:.1e}", flush=True)
+    assert err < 1e-10
+    assert np.array_equal(np.asarray(nmt.mask_apodization(np.ones(12 * nside * nside), apo, "Smooth")),
+                          np.ones(12 * nside * nside))
