@@ -78,6 +78,16 @@ def test_nchunks():
         ("gm_toeplitz", (1, 10, 10, 3, 2, 2, None), 1),                     # l_toeplitz = lmax
         ("gm_toeplitz", (1, 10, 6, 7, 2, 2, None), 1),                      # l_exact > l_toeplitz
         ("gm_toeplitz", (1, 10, 6, 3, 2, 1, None), 1),                      # out aliases M
+        ("gm_gauss_legendre", (0, 1, 1, None), 1),                          # order 0
+        ("gm_wigner_d_table", (1, 10, 17, 0, 5, None, 1, None), 1),         # |m| > 16
+        ("gm_wigner_d_corr", (1, 10, 0, 0, -1, 1, 1, None, None), 1),       # negative lmax
+        ("gm_table_combine", (9, 1, None, None, 1, None, 4, 4, 1, None), 1),  # more than 8 terms
+        ("gm_parity_split", (1, 1, 0, 1, 1, None), 1),                      # L = 0
+        ("gm_mask_apodize", (1, 12, 0.1, 1, 1, 1, None), 1),                # nside not a power of 2
+        ("gm_mask_apodize", (1, 16, 0.1, 4, 1, 1, None), 1),                # unknown kind
+        ("gm_mask_apodize", (1, 16, 4.0, 1, 1, 1, None), 1),                # scale beyond pi
+        ("gm_gaussian_alms", (1, 10, 9, 1, 1, None), 1),                    # 9 components
+        ("gm_gauss_draws", (1, 0, 1, 1, None), 1),                          # no elements
     ],
 )
 def test_argument_validation(name, args, code):
