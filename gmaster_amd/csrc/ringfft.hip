@@ -605,8 +605,17 @@ __global__ void k_ring_inv_s_split(const double2* __restrict__ spec, int64_t ld,
 
 // ----------------------------------------------------------------------- fold residual
 // One workgroup per ring, one thread per residue class c of m modulo nphi.
-__global__ void k_fold_s0(const double2* __restrict__ F, const double2* G, double2* R, int64_t ld,
-                          int64_t map_stride, int nside, int L) {
+// Rings with fewer residue classes than threads (the polar caps: nphi = 4 i) give each class
+// tpc = blockDim / nres threads: the class sum is split over them and reduced through LDS,
+// and the writes of the class are shared the same way.  Larger rings keep one thread per
+// class.  The branch depends only on the ring, so every barrier is reached by the whole
+// workgroup.  Each output element k is read from G and written to R by the same thread, so
+// R may alias G.
+constexpr int kFoldThreads = 256;
+
+__global__ void __launch_bounds__(kFoldThreads) k_fold_s0(const double2* __restrict__ F, const double2* G, double2* R,
+                                                          int64_t ld, int64_t map_stride, int nside, int L) {
+    __shared__ double2 part[kFoldThreads];
     const int r = blockIdx.x;
     const Ring g = ring_info(nside, r);
     const int n = g.nphi;
@@ -616,28 +625,57 @@ __global__ void k_fold_s0(const double2* __restrict__ F, const double2* G, doubl
     const double2* gin = G + blockIdx.y * map_stride + (int64_t)r * ld;
     double2* out = R + blockIdx.y * map_stride + (int64_t)r * ld;
     const int nres = n < L ? n : L;
-    for (int c = threadIdx.x; c < nres; c += blockDim.x) {
-        double2 s = make_double2(0.0, 0.0);
-        // m = c + j n over (-L, L); F[-m] = conj F[m]
-        int64_t jlo = -(((int64_t)L - 1 + c) / n);
-        for (int64_t j = jlo; c + j * n < L; ++j) {
+    const int tpc = nres < kFoldThreads ? kFoldThreads / nres : 1;
+    if (tpc == 1) {
+        for (int c = threadIdx.x; c < nres; c += blockDim.x) {
+            double2 s = make_double2(0.0, 0.0);
+            // m = c + j n over (-L, L); F[-m] = conj F[m]
+            const int64_t jlo = -(((int64_t)L - 1 + c) / n);
+            for (int64_t j = jlo; c + j * n < L; ++j) {
+                const int64_t m = c + j * n;
+                double2 v = (m >= 0) ? f[m] : cconj(f[-m]);
+                if (sigma < 0 && (j & 1)) v = make_double2(-v.x, -v.y);
+                s = cadd(s, v);
+            }
+            s = cscale(s, scale);
+            for (int64_t j = 0; c + j * n < L; ++j) {
+                const int64_t k = c + j * n;
+                const double sg = (sigma < 0 && (j & 1)) ? -1.0 : 1.0;
+                const double2 gk = gin[k];
+                out[k] = make_double2(sg * s.x - gk.x, sg * s.y - gk.y);
+            }
+        }
+        return;
+    }
+    const int c = threadIdx.x / tpc, u = threadIdx.x % tpc;
+    double2 s = make_double2(0.0, 0.0);
+    if (c < nres) {
+        const int64_t jlo = -(((int64_t)L - 1 + c) / n);
+        for (int64_t j = jlo + u; c + j * n < L; j += tpc) {
             const int64_t m = c + j * n;
             double2 v = (m >= 0) ? f[m] : cconj(f[-m]);
             if (sigma < 0 && (j & 1)) v = make_double2(-v.x, -v.y);
             s = cadd(s, v);
         }
-        s = cscale(s, scale);
-        for (int64_t j = 0; c + j * n < L; ++j) {
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (c < nres) {
+        double2 t = make_double2(0.0, 0.0);
+        for (int i = 0; i < tpc; ++i) t = cadd(t, part[c * tpc + i]);  // fixed order: deterministic
+        t = cscale(t, scale);
+        for (int64_t j = u; c + j * n < L; j += tpc) {
             const int64_t k = c + j * n;
             const double sg = (sigma < 0 && (j & 1)) ? -1.0 : 1.0;
             const double2 gk = gin[k];
-            out[k] = make_double2(sg * s.x - gk.x, sg * s.y - gk.y);
+            out[k] = make_double2(sg * t.x - gk.x, sg * t.y - gk.y);
         }
     }
 }
 
-__global__ void k_fold_s(const double2* __restrict__ F, const double2* G, double2* R, int64_t ld,
-                         int nside, int L) {
+__global__ void __launch_bounds__(kFoldThreads) k_fold_s(const double2* __restrict__ F, const double2* G, double2* R,
+                                                         int64_t ld, int nside, int L) {
+    __shared__ double2 part[kFoldThreads];
     const int r = blockIdx.x;
     const Ring g = ring_info(nside, r);
     const int n = g.nphi;
@@ -647,22 +685,49 @@ __global__ void k_fold_s(const double2* __restrict__ F, const double2* G, double
     const double2* f = F + (int64_t)r * ld + col0;
     const double2* gin = G + (int64_t)r * ld + col0;
     double2* out = R + (int64_t)r * ld + col0;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) {
-        // all m = c + j n with |m| < L
-        const int64_t jlo = -(((int64_t)L - 1 + c) / n);
-        if (c + jlo * n >= L) continue;
-        double2 s = make_double2(0.0, 0.0);
-        for (int64_t j = jlo; c + j * n < L; ++j) {
+    const int tpc = n < kFoldThreads ? kFoldThreads / n : 1;
+    if (tpc == 1) {
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            // all m = c + j n with |m| < L
+            const int64_t jlo = -(((int64_t)L - 1 + c) / n);
+            if (c + jlo * n >= L) continue;
+            double2 s = make_double2(0.0, 0.0);
+            for (int64_t j = jlo; c + j * n < L; ++j) {
+                double2 v = f[c + j * n];
+                if (sigma < 0 && (j & 1)) v = make_double2(-v.x, -v.y);
+                s = cadd(s, v);
+            }
+            s = cscale(s, scale);
+            for (int64_t j = jlo; c + j * n < L; ++j) {
+                const int64_t k = c + j * n;
+                const double sg = (sigma < 0 && (j & 1)) ? -1.0 : 1.0;
+                const double2 gk = gin[k];
+                out[k] = make_double2(sg * s.x - gk.x, sg * s.y - gk.y);
+            }
+        }
+        return;
+    }
+    const int c = threadIdx.x / tpc, u = threadIdx.x % tpc;
+    const int64_t jlo = (c < n) ? -(((int64_t)L - 1 + c) / n) : 0;
+    double2 s = make_double2(0.0, 0.0);
+    if (c < n) {
+        for (int64_t j = jlo + u; c + j * n < L; j += tpc) {
             double2 v = f[c + j * n];
             if (sigma < 0 && (j & 1)) v = make_double2(-v.x, -v.y);
             s = cadd(s, v);
         }
-        s = cscale(s, scale);
-        for (int64_t j = jlo; c + j * n < L; ++j) {
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (c < n) {
+        double2 t = make_double2(0.0, 0.0);
+        for (int i = 0; i < tpc; ++i) t = cadd(t, part[c * tpc + i]);
+        t = cscale(t, scale);
+        for (int64_t j = jlo + u; c + j * n < L; j += tpc) {
             const int64_t k = c + j * n;
             const double sg = (sigma < 0 && (j & 1)) ? -1.0 : 1.0;
             const double2 gk = gin[k];
-            out[k] = make_double2(sg * s.x - gk.x, sg * s.y - gk.y);
+            out[k] = make_double2(sg * t.x - gk.x, sg * t.y - gk.y);
         }
     }
 }
@@ -838,7 +903,7 @@ extern "C" int gm_ring_fold_residual(const double* F, const double* G, double* R
     if (ld < L) return fail(GM_E_SHAPE, "gm_ring_fold_residual: ld < lmax + 1");
     if (nmaps < 1 || (nmaps > 1 && map_stride < nring * ld))
         return fail(GM_E_SHAPE, "gm_ring_fold_residual: bad nmaps / map_stride");
-    hipLaunchKernelGGL(k_fold_s0, dim3((unsigned)nring, (unsigned)nmaps), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_fold_s0, dim3((unsigned)nring, (unsigned)nmaps), dim3(kFoldThreads), 0, (hipStream_t)stream,
                        reinterpret_cast<const double2*>(F), reinterpret_cast<const double2*>(G),
                        reinterpret_cast<double2*>(R), ld, map_stride, (int)nside, (int)L);
     return check_launch("gm_ring_fold_residual");
@@ -850,7 +915,7 @@ extern "C" int gm_ring_fold_residual_c(const double* F, const double* G, double*
     if (!F || !G || !R) return fail(GM_E_ARG, "gm_ring_fold_residual_c: null pointer");
     const int64_t L = lmax + 1, nring = 4 * nside - 1;
     if (ld < 2 * L - 1) return fail(GM_E_SHAPE, "gm_ring_fold_residual_c: ld < 2 lmax + 1");
-    hipLaunchKernelGGL(k_fold_s, dim3((unsigned)nring), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_fold_s, dim3((unsigned)nring), dim3(kFoldThreads), 0, (hipStream_t)stream,
                        reinterpret_cast<const double2*>(F), reinterpret_cast<const double2*>(G),
                        reinterpret_cast<double2*>(R), ld, (int)nside, (int)L);
     return check_launch("gm_ring_fold_residual_c");
