@@ -11,7 +11,9 @@ Restates ``gmaster/workspaces.py``:
 * the slot / sign layout of the blocks (``:1851-1903``), binning with weights, f_ell and
   beams (``:1452-1470, :292-310``) -- kernel ``gm_bin_block``;
 * ``decouple_cell`` (``:2106-2147``): binned spectra (``gm_bin_cell``) and an FP64 LU
-  solve of the binned MCM (``torch.linalg.solve``, LAPACK-style getrf/getrs on the GPU).
+  solve of the binned MCM: its LU factors (``torch.linalg.lu_factor``, getrf on the GPU)
+  are computed once per binned matrix and reused by every ``decouple_cell`` and by the
+  bandpower windows (getrs).
 """
 
 import hashlib
@@ -524,6 +526,7 @@ class NmtWorkspace:
             mb = self.wawb * torch.eye(nb * ncls, dtype=torch.float64, device=dev)
         self.mcm_binned = mb
         self._bpws = None
+        self._lu = None
 
     # ------------------------------------------------------------------ products
     def decouple_cell(self, cl_in, cl_bias=None, cl_noise=None):
@@ -541,8 +544,14 @@ class NmtWorkspace:
                 total = total - b[:, : self.lmax + 1]
         binned = self.bins.bin_cell(total.contiguous()).as_subclass(torch.Tensor)
         rhs = binned.T.reshape(-1)
-        sol = torch.linalg.solve(self.mcm_binned, rhs)
+        sol = self._solve(rhs[:, None])[:, 0]
         return wrap(sol.reshape(self.nbands, self.ncls).T.contiguous())
+
+    def _solve(self, rhs):
+        """``mcm_binned^-1 rhs`` from LU factors computed once per binned matrix."""
+        if getattr(self, "_lu", None) is None:
+            self._lu = torch.linalg.lu_factor(self.mcm_binned)
+        return torch.linalg.lu_solve(*self._lu, rhs)
 
     def couple_cell(self, cl_in):
         """Theory ``(ncls, >= lmax+1)`` -> expected coupled spectra ``(ncls, lmax+1)``."""
@@ -577,7 +586,7 @@ class NmtWorkspace:
             one = torch.zeros((nb, ncls, L, ncls), dtype=torch.float64, device=dev)
             for name, (c1, c2), sign in self.layout:
                 one[:, c1, :, c2] += sign * (wbin @ self.blocks[name]) * beam[None, :]
-            sol = torch.linalg.solve(self.mcm_binned, one.reshape(nb * ncls, L * ncls))
+            sol = self._solve(one.reshape(nb * ncls, L * ncls))
             self._bpws = sol.reshape(nb, ncls, L, ncls).permute(1, 0, 3, 2).contiguous()
         return wrap(self._bpws)
 
