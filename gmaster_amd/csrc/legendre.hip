@@ -949,18 +949,23 @@ constexpr int large_r(bool syn, bool mixed, bool spin0, int K = 1) {
 }
 
 // Ring counts up to GM_MID_RINGS use 2-wave workgroups of 4 rings per lane: more, shorter
-// workgroups for the smaller geometries.
+// workgroups for the smaller geometries.  The spin-0 analyses and the spin-s syntheses keep
+// them up to kMidRingsWide rings (measured per pass at Nside 512 and 1024: 25-35% faster
+// there, while the spin-s analyses and spin-0 syntheses are slower with them above 512).
 #ifndef GM_MID_RINGS
 #define GM_MID_RINGS 512
 #endif
+constexpr int64_t kMidRingsWide = 2048;
 
 LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn, int K = 1) {
     const int64_t nrings = spin == 0 ? 2 * nside : 4 * nside - 1;
     LatConfig cfg;
     // the launchers select the instantiation by NW: NW 1 -> R 2, NW 2 -> R 4, NW 4 -> the
     // large R; CH = 64 NW R must match the kernel's own chunk size
+    const bool wide = syn ? spin != 0 : spin == 0;
+    const int64_t mid = wide && kMidRingsWide > GM_MID_RINGS ? kMidRingsWide : GM_MID_RINGS;
     if (nrings <= 128) cfg = {2, 1, 128, 0, 0};
-    else if (nrings <= GM_MID_RINGS) cfg = {4, 2, 512, 0, 0};
+    else if (nrings <= mid) cfg = {4, 2, 512, 0, 0};
     else {
         const int r = large_r(syn, mixed, spin == 0, K);
         cfg = {r, 4, 256 * r, 0, 0};
