@@ -955,6 +955,10 @@ constexpr int large_r(bool syn, bool mixed, bool spin0, int K = 1) {
 #ifndef GM_MID_RINGS
 #define GM_MID_RINGS 512
 #endif
+// (the spin-s syntheses measured slower with them at 4095 rings, Nside 1024: 12.1 vs 10.2 ms)
+#ifndef GM_MID_RINGS_SYN_S
+#define GM_MID_RINGS_SYN_S 2048
+#endif
 constexpr int64_t kMidRingsWide = 2048;
 
 LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn, int K = 1) {
@@ -963,7 +967,8 @@ LatConfig make_config(int64_t nside, int64_t spin, bool mixed, bool syn, int K =
     // the launchers select the instantiation by NW: NW 1 -> R 2, NW 2 -> R 4, NW 4 -> the
     // large R; CH = 64 NW R must match the kernel's own chunk size
     const bool wide = syn ? spin != 0 : spin == 0;
-    const int64_t mid = wide && kMidRingsWide > GM_MID_RINGS ? kMidRingsWide : GM_MID_RINGS;
+    const int64_t wide_rings = (syn && spin != 0) ? GM_MID_RINGS_SYN_S : kMidRingsWide;
+    const int64_t mid = wide && wide_rings > GM_MID_RINGS ? wide_rings : GM_MID_RINGS;
     if (nrings <= 128) cfg = {2, 1, 128, 0, 0};
     else if (nrings <= mid) cfg = {4, 2, 512, 0, 0};
     else {
