@@ -55,6 +55,7 @@ SIGNATURES = {
     "gm_toeplitz": [_p, _i64, _i64, _i64, _i64, _p, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
+    "gm_fingerprint": [_p, _i64, _p, _p],
     "gm_map_dots_work_size": [_i64, _i64, _i64],
     "gm_map_dots": [_p, _i64, _i64, _p, _i64, _i64, _i64, _p, _i64, _d, _p, _p, _p],
     "gm_map_deproject": [_p, _i64, _p, _i64, _i64, _p, _p],

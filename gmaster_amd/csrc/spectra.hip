@@ -114,3 +114,67 @@ extern "C" int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, 
                        (hipStream_t)stream, M, (int)(lmax + 1), ells, band_start, (int)nbands, wout, tin, out);
     return check_launch("gm_bin_block");
 }
+
+// ------------------------------------------------------------------ content fingerprint
+namespace gm {
+namespace {
+
+// splitmix64 finaliser (Steele, Lea & Flood 2014): a bijective 64-bit mixer.
+__device__ inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ void k_fingerprint_clear(unsigned long long* out) {
+    if (threadIdx.x < 2) out[threadIdx.x] = 0ull;
+}
+
+// Two order-independent 64-bit sums of position-keyed word hashes; any change of one word
+// changes both sums unless two independent 64-bit hashes collide.
+__global__ void k_fingerprint(const uint64_t* __restrict__ w, int64_t nwords, const unsigned char* __restrict__ tail,
+                              int ntail, unsigned long long* out) {
+    uint64_t s1 = 0, s2 = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nwords; i += stride) {
+        const uint64_t h = mix64(w[i] ^ mix64((uint64_t)i + 0x9e3779b97f4a7c15ull));
+        s1 += h;
+        s2 += mix64(h ^ 0x632be59bd9b4e019ull);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ntail > 0) {
+        uint64_t t = 0;
+        for (int k = 0; k < ntail; ++k) t |= (uint64_t)tail[k] << (8 * k);
+        const uint64_t h = mix64(t ^ mix64((uint64_t)nwords + 0x9e3779b97f4a7c15ull) ^ 0xff51afd7ed558ccdull);
+        s1 += h;
+        s2 += mix64(h ^ 0x632be59bd9b4e019ull);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s1 += __shfl_down(s1, off);
+        s2 += __shfl_down(s2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&out[0], (unsigned long long)s1);
+        atomicAdd(&out[1], (unsigned long long)s2);
+    }
+}
+
+}  // namespace
+}  // namespace gm
+
+extern "C" int gm_fingerprint(const void* data, int64_t nbytes, uint64_t* out, void* stream) {
+    if (!out || (nbytes > 0 && !data)) return fail(GM_E_ARG, "gm_fingerprint: null pointer");
+    if (nbytes < 0) return fail(GM_E_ARG, "gm_fingerprint: nbytes < 0");
+    if (reinterpret_cast<uintptr_t>(data) & 7) return fail(GM_E_ARG, "gm_fingerprint: data not 8-byte aligned");
+    hipStream_t st = (hipStream_t)stream;
+    auto* o = reinterpret_cast<unsigned long long*>(out);
+    hipLaunchKernelGGL(k_fingerprint_clear, dim3(1), dim3(64), 0, st, o);
+    const int64_t nwords = nbytes / 8;
+    const int ntail = (int)(nbytes - nwords * 8);
+    int64_t blocks = (nwords + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fingerprint, dim3((unsigned)blocks), dim3(256), 0, st,
+                       reinterpret_cast<const uint64_t*>(data), nwords,
+                       reinterpret_cast<const unsigned char*>(data) + nwords * 8, ntail, o);
+    return check_launch("gm_fingerprint");
+}

@@ -19,14 +19,25 @@ from .purify import purify
 from ._config import nmt_params
 from .utils import NmtAlmInfo, NmtMapInfo
 
-# Mask alms shared by fields built on the same, unmodified mask tensor (keyed on its storage,
-# in-place version counter, band limit and n_iter), so that every workspace of one mask
-# sees bit-identical mask spectra and shares one coupling computation.
+# Mask alms shared by fields built on masks with the same content (a 128-bit fingerprint of
+# the mask's bytes, computed on the device by gm_fingerprint), band limit, n_iter and
+# precision, so that every workspace of one mask sees bit-identical mask spectra and shares
+# one coupling computation.  Keying on content rather than on the tensor's address keeps a
+# new mask that the caching allocator places at a freed mask's address from hitting the
+# freed mask's alms.
 _MASK_ALMS = {}
 
 
+def mask_fingerprint(mask):
+    """128-bit content fingerprint of a contiguous device tensor (two Python ints)."""
+    out = torch.empty(2, dtype=torch.int64, device=mask.device)
+    _lib.call("gm_fingerprint", _lib.ptr(mask), mask.numel() * mask.element_size(), _lib.ptr(out),
+              _lib.stream_ptr(mask.device))
+    return tuple(int(v) for v in out.cpu())
+
+
 def _mask_key(mask, lmax_mask, n_iter_mask, precision=None):
-    return (mask.data_ptr(), mask._version, mask.numel(), str(mask.device), int(lmax_mask), int(n_iter_mask),
+    return (mask_fingerprint(mask), mask.numel(), str(mask.device), int(lmax_mask), int(n_iter_mask),
             precision or _precision())
 
 

@@ -356,12 +356,7 @@ class NmtWorkspace:
         L = self.lmax + 1
         ell = np.arange(L, dtype=np.int32)
         b = self.bins
-        bands = np.full(L, -1, dtype=np.int32)
-        weights = np.zeros(L)
-        f_ell = np.zeros(L)
-        bands[b._ells_np] = b._bpws_np
-        weights[b._ells_np] = b._weights_np
-        f_ell[b._ells_np] = b._f_ell_np
+        bands, weights, f_ell = b.per_ell_table()
         pcl = self.pcl_mask.detach().cpu().numpy().astype(np.float64)
         fits.write(fname, self.get_coupling_matrix().cpu().numpy(), header, [
             ("BEAMS", [("L", ell), ("BEAM1", self.beam1.cpu().numpy().astype(np.float64)),
@@ -507,7 +502,7 @@ class NmtWorkspace:
 
     def _postprocess(self):
         dev = self.blocks["00"].device
-        d = self.bins._device(dev)
+        d = self.bins.device_view(dev)
         beam = (self.beam1 * self.beam2).to(dev)
         tin = (beam[d["ells"].long()] * d["inv_f"]).contiguous()
         nb, ncls = self.nbands, self.ncls
@@ -578,10 +573,7 @@ class NmtWorkspace:
         if self._bpws is None:
             dev = self.mcm_binned.device
             L, nb, ncls = self.lmax + 1, self.nbands, self.ncls
-            wbin = torch.zeros((nb, L), dtype=torch.float64, device=dev)
-            d = self.bins._device(dev)
-            bp = torch.as_tensor(self.bins._bpws_np, dtype=torch.long, device=dev)
-            wbin[bp, d["ells"].long()] = d["w"]
+            wbin = self.bins.binning_matrix(dev)
             beam = self.beam1 * self.beam2
             one = torch.zeros((nb, ncls, L, ncls), dtype=torch.float64, device=dev)
             for name, (c1, c2), sign in self.layout:
@@ -627,6 +619,62 @@ class NmtWorkspace:
         self._postprocess()
 
 
+def _coefficient_blocks(spin1, spin2, is_teb):
+    """Names of the distinct blocks a spin pair couples through: the spin-0 pair ("00"), the
+    spin-0 x spin-s pair ("0s") and the spin-s pair's M+ / M- ("pp", "mm").  A joint TEB
+    workspace has all four."""
+    zero1, zero2 = spin1 == 0, spin2 == 0
+    names = set()
+    if is_teb or (zero1 and zero2):
+        names.add("00")
+    if is_teb or zero1 != zero2:
+        names.add("0s")
+    if is_teb or not (zero1 or zero2):
+        names.update(("pp", "mm"))
+    return names
+
+
+def _check_coefficient_request(spin1, spin2, is_teb, pure_any):
+    rules = (
+        (is_teb and (spin1 != 0 or spin2 == 0), "is_teb is only valid for spin1=0 and spin2!=0"),
+        (pure_any and 2 not in (spin1, spin2), "pure_any is only valid for spin 2"),
+        (pure_any and not {spin1, spin2} <= {0, 2}, "Purification is only implemented for spin 2"),
+    )
+    for broken, message in rules:
+        if broken:
+            raise ValueError(message)
+
+
+def _mask_coefficients(window, lmax, spin1, spin2, is_teb, pure_any, toe):
+    """Blocks of one mask window (already cut or zero-padded to 2 lmax + 1 entries), each
+    divided by 2 l' + 1; "0s", "pp", "mm" carry the purification-level axis."""
+    blocks = dict(coupling_blocks(window, lmax, toe))
+    if not {spin1, spin2} <= {0, 2}:
+        # other spins: the 0-s block pairs the non-zero spin with 0, the s x s blocks use
+        # (spin1, spin2), or (spin2, spin2) for a joint TEB pair (gmaster/workspaces.py:1279-1316)
+        names = _coefficient_blocks(spin1, spin2, is_teb)
+        if "0s" in names:
+            blocks.update(general_spin_blocks(window, lmax, *((0, spin2) if spin1 == 0 else (spin1, 0)),
+                                              toeplitz=toe))
+        if "pp" in names:
+            gen = general_spin_blocks(window, lmax, spin2 if is_teb else spin1, spin2, toeplitz=toe)
+            blocks["22p"], blocks["22m"] = gen["22p"], gen["22m"]
+    inv_cols = 1.0 / (2 * torch.arange(lmax + 1, dtype=torch.float64, device=window.device) + 1)
+    levels = {"0s": [blocks["02"]], "pp": [blocks["22p"]], "mm": [blocks["22m"]]}
+    if pure_any:
+        # purification levels 0, 1, 2 (gmaster/workspaces.py:1317-1325)
+        from .purecoupling import pure_coupling_blocks
+
+        pure = pure_coupling_blocks(window, lmax)
+        levels["0s"].append(pure["02pure"])
+        levels["pp"] += [pure["22p1"], pure["22p2"]]
+        levels["mm"] += [pure["22m1"], pure["22m2"]]
+    res = {"00": blocks["00"] * inv_cols}
+    for name, mats in levels.items():
+        res[name] = torch.stack(mats) * inv_cols
+    return res
+
+
 def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any=False, l_toeplitz=-1, l_exact=-1,
                             dl_band=-1):
     """Distinct MASTER coupling blocks of a spin pair, divided by ``2 l' + 1``
@@ -636,59 +684,32 @@ def get_master_coefficients(pcl_mask, lmax, spin1, spin2, is_teb=False, pure_any
     before the last two when ``pcl_mask`` is 2D.  Spin pairs outside {0, 2} take their 0-s and
     s x s blocks from the general quadrature (``general_spin_blocks``)."""
     spin1, spin2, lmax = int(spin1), int(spin2), int(lmax)
-    if is_teb and not (spin1 == 0 and spin2 != 0):
-        raise ValueError("is_teb is only valid for spin1=0 and spin2!=0")
-    if pure_any and spin1 != 2 and spin2 != 2:
-        raise ValueError("pure_any is only valid for spin 2")
-    if pure_any and (spin1 not in (0, 2) or spin2 not in (0, 2)):
-        raise ValueError("Purification is only implemented for spin 2")
+    _check_coefficient_request(spin1, spin2, is_teb, pure_any)
     toeplitz_sanity(l_toeplitz, l_exact, dl_band, lmax)
-    masks = device_tensor(pcl_mask)
-    one_dimensional = masks.ndim == 1
-    if one_dimensional:
-        masks = masks[None]
-    if masks.ndim != 2:
+    windows = device_tensor(pcl_mask)
+    if windows.ndim not in (1, 2):
         raise ValueError("pcl_mask must be one- or two-dimensional")
-    lmax_mask = masks.shape[1] - 1
-    dev = masks.device
-    cols = (2 * torch.arange(lmax + 1, dtype=torch.float64, device=dev) + 1)[None]
-    has_00 = (spin1 == spin2 == 0) or is_teb
-    has_0s = ((spin1 != spin2) and (spin1 == 0 or spin2 == 0)) or is_teb
-    has_ss = (spin1 != 0 and spin2 != 0) or is_teb
+    stacked = windows.ndim == 2
+    windows = windows.reshape(-1, windows.shape[-1])
+    nw = windows.shape[1]
+    # every window cut or zero-padded to the 2 lmax + 1 multipoles the 3j sums reach
+    cut = torch.zeros((windows.shape[0], 2 * lmax + 1), dtype=torch.float64, device=windows.device)
+    keep = min(nw, 2 * lmax + 1)
+    cut[:, :keep] = windows[:, :keep]
     toe = (l_toeplitz, l_exact, dl_band) if l_toeplitz > 0 else None
-    per_mask = []
-    for m in masks:
-        padded = torch.zeros(2 * lmax + 1, dtype=torch.float64, device=dev)
-        n = min(len(m), 2 * lmax + 1)
-        padded[:n] = m[:n]
-        b = dict(coupling_blocks(padded, lmax, toe))
-        if spin1 not in (0, 2) or spin2 not in (0, 2):
-            # the 0-s block pairs the non-zero spin with 0; s x s blocks use (spin1, spin2)
-            # or (spin2, spin2) for a joint TEB pair (``gmaster/workspaces.py:1279-1316``)
-            if has_0s:
-                pair = (0, spin2) if spin1 == 0 else (spin1, 0)
-                b.update(general_spin_blocks(m, lmax, pair[0], pair[1], toeplitz=toe))
-            if has_ss:
-                ss1 = spin2 if is_teb else spin1
-                gb = general_spin_blocks(m, lmax, ss1, spin2, toeplitz=toe)
-                b["22p"], b["22m"] = gb["22p"], gb["22m"]
-        entry = {"00": b["00"] / cols, "0s": (b["02"] / cols)[None], "pp": (b["22p"] / cols)[None],
-                 "mm": (b["22m"] / cols)[None]}
-        if pure_any:
-            # levels 0, 1, 2 along the leading axis (``gmaster/workspaces.py:1317-1325``)
-            from .purecoupling import pure_coupling_blocks
-            p = pure_coupling_blocks(padded, lmax)
-            entry["0s"] = torch.cat([entry["0s"], (p["02pure"] / cols)[None]])
-            entry["pp"] = torch.cat([entry["pp"], (p["22p1"] / cols)[None], (p["22p2"] / cols)[None]])
-            entry["mm"] = torch.cat([entry["mm"], (p["22m1"] / cols)[None], (p["22m2"] / cols)[None]])
-        per_mask.append(entry)
+    per_window = [_mask_coefficients(w, lmax, spin1, spin2, is_teb, pure_any, toe) for w in cut]
+    present = _coefficient_blocks(spin1, spin2, is_teb)
     out = {}
-    for name, present in (("00", has_00), ("0s", has_0s), ("pp", has_ss), ("mm", has_ss)):
-        if not present:
+    for name in ("00", "0s", "pp", "mm"):
+        if name not in present:
             out[name] = None
-            continue
-        arrays = [v[name] for v in per_mask]
-        out[name] = wrap(arrays[0] if one_dimensional else torch.stack(arrays, dim=-3))
-    out.update(pure_any=bool(pure_any), toeplitz={"l_toeplitz": l_toeplitz, "l_exact": l_exact, "dl_band": dl_band},
-               spins=(spin1, spin2), lmax=lmax, lmax_mask=lmax_mask)
+        elif stacked:
+            out[name] = wrap(torch.stack([c[name] for c in per_window], dim=-3))
+        else:
+            out[name] = wrap(per_window[0][name])
+    out["pure_any"] = bool(pure_any)
+    out["toeplitz"] = {"l_toeplitz": l_toeplitz, "l_exact": l_exact, "dl_band": dl_band}
+    out["spins"] = (spin1, spin2)
+    out["lmax"] = lmax
+    out["lmax_mask"] = nw - 1
     return out

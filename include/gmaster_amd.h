@@ -192,6 +192,13 @@ int gm_bin_block(const double* M, int64_t lmax, const int32_t* ells, const int32
 int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells,
                 const int32_t* band_start, int64_t nbands, const double* w, double* out, void* stream);
 
+/* 128-bit content fingerprint of a device buffer (two 64-bit sums of position-keyed
+ * splitmix64 word hashes, written to out[0..1]; data 8-byte aligned).  Keys the mask-alms
+ * cache on the mask's bytes, so a new mask that lands at a freed mask's address never
+ * reuses its alms.  No reference counterpart: the reference recomputes mask alms per field
+ * (gmaster/field.py:273-289, 348-376). */
+int gm_fingerprint(const void* data, int64_t nbytes, uint64_t* out, void* stream);
+
 /* ---------------------------------------------------------------- contaminant deprojection */
 
 /* Weighted inner products of flattened maps (each vector n doubles, e.g. nmaps x npix):

@@ -35,6 +35,8 @@ def load():
     lib.gmo_lat_ana_s.restype = None
     lib.gmo_lat_ana_s_cols.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, p, p, p, ctypes.c_int, p]
     lib.gmo_lat_ana_s_cols.restype = None
+    lib.gmo_lat_syn_s.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, p, p, p, ctypes.c_int, p]
+    lib.gmo_lat_syn_s.restype = None
     lib.gmo_coupling.argtypes = [ctypes.c_int, p, ctypes.c_int, p, p, ctypes.c_int, p]
     lib.gmo_coupling.restype = None
     return lib
@@ -65,6 +67,23 @@ def lat_ana_s_cols(nside, lmax, spin, dcol, mcol, orders):
     out = np.zeros((len(orders), L, 4))
     lib.gmo_lat_ana_s_cols(nside, lmax, spin, dcol.ctypes.data, mcol.ctypes.data, orders.ctypes.data, len(orders),
                            out.ctypes.data)
+    return out
+
+
+def lat_syn_s(nside, lmax, spin, cd, cm, orders):
+    """``(norders, nring, 4)``: per ring, direct ``sum_l cd[k][l] lambda_lm(theta_r)`` and
+    mirror ``sum_l cm[k][l] lambda_lm(pi - theta_r)`` (re, im); ``cd``, ``cm`` are
+    ``(len(orders), L)`` complex."""
+    lib = load()
+    L = lmax + 1
+    cd = np.ascontiguousarray(cd, dtype=np.complex128)
+    cm = np.ascontiguousarray(cm, dtype=np.complex128)
+    orders = np.ascontiguousarray(orders, dtype=np.int32)
+    if cd.shape != (len(orders), L) or cm.shape != cd.shape:
+        raise ValueError("coefficients must be (len(orders), lmax + 1)")
+    out = np.zeros((len(orders), 4 * nside - 1, 4))
+    lib.gmo_lat_syn_s(nside, lmax, spin, cd.ctypes.data, cm.ctypes.data, orders.ctypes.data, len(orders),
+                      out.ctypes.data)
     return out
 
 

@@ -7,6 +7,11 @@
  *                  a_M[l] = sum_r lambda_lm(theta_r) G[mirror r][-m]
  *                  (restates gmaster/_native/cuda/march_v2.cu:155-285 in FP64, with the
  *                  recurrence and seeds of docs/notes/latitudinal_march_maths.md:28-128)
+ *  gmo_lat_syn_s   spin-s Legendre synthesis per order m over all rings, direct and mirror:
+ *                  D[r] = sum_l cd[l] lambda_lm(theta_r),
+ *                  Mi[r] = sum_l cm[l] lambda_lm(pi - theta_r)
+ *                  (restates gmaster/_native/cuda/march_v2.cu:291-407 in FP64; the same
+ *                  natural-form recurrence as the analysis)
  *  gmo_coupling    the four MASTER coupling sums per (l1, l2 >= l1) from the Wigner-3j
  *                  rows (l1 l2 l3; 0 0 0), (l1 l2 l3; 2 -2 0) (gmaster/workspaces.py:1069-1077)
  *
@@ -52,6 +57,79 @@ static double seed_log2_const(int m, int s) {
     return 0.5 * log2((2.0 * M + 1.0) / (4.0 * M_PI)) + 0.5 * lb;
 }
 
+/* Natural-form coefficients of order m, spin s:
+ * lambda_{l+1} = (A[l+1] z + C[l+1]) lambda_l - B[l+1] lambda_{l-1}. */
+static void rec_coefficients(int L, int m, int s, double* A, double* C, double* B) {
+    const int M = m > s ? m : s;
+    for (int lp = M + 1; lp < L; ++lp) {
+        const int l = lp - 1;
+        const double K = (l + 1.0) * (2.0 * l + 1.0) /
+                         sqrt(((l + 1.0) * (l + 1.0) - (double)m * m) * ((l + 1.0) * (l + 1.0) - (double)s * s));
+        A[lp] = sqrt((2.0 * l + 3.0) / (2.0 * l + 1.0)) * K;
+        C[lp] = l > 0 ? A[lp] * m * s / ((double)l * (l + 1.0)) : 0.0;
+        B[lp] = l > 0 ? sqrt((2.0 * l + 3.0) / (2.0 * l - 1.0)) * K *
+                            sqrt(((double)l * l - (double)m * m) * ((double)l * l - (double)s * s)) /
+                            ((double)l * (2.0 * l + 1.0))
+                      : 0.0;
+    }
+}
+
+/* One order m, synthesis: for every ring q, lambda_lm(theta_q) for l = M..L-1 feeds the
+ * direct sum of ring q and the mirror sum of ring nring-1-q (whose reflection is q).
+ * out[4 r + (dre, dim, mre, mim)]; cd, cm complex (re, im) per degree. */
+static void syn_one(int nside, int lmax, int s, int m, const double* cd, const double* cm, double* out) {
+    const int L = lmax + 1;
+    const int nring = 4 * nside - 1;
+    const int M = m > s ? m : s;
+    for (int k = 0; k < 4 * nring; ++k) out[k] = 0.0;
+    if (M >= L) return;
+    double* A = (double*)malloc(sizeof(double) * 3 * L);
+    double* C = A + L;
+    double* B = A + 2 * L;
+    rec_coefficients(L, m, s, A, C, B);
+    const double c0 = seed_log2_const(m, s);
+    const double sign = ((m + s) & 1) ? -1.0 : 1.0;
+    for (int q = 0; q < nring; ++q) {
+        const ring_t g = ring_geom(nside, q);
+        const int qm = nring - 1 - q;
+        double dre = 0, dim = 0, mre = 0, mim = 0;
+        const double f = c0 + (m + s) * log2(g.shalf) + abs(m - s) * log2(g.chalf);
+        int e = (int)floor(f);
+        double v = sign * exp2(f - e), vp = 0.0;
+        double scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
+        for (int l = M; l < L; ++l) {
+            if (scale != 0.0) {
+                const double lam = v * scale;
+                dre += lam * cd[2 * l];
+                dim += lam * cd[2 * l + 1];
+                mre += lam * cm[2 * l];
+                mim += lam * cm[2 * l + 1];
+            }
+            if (l + 1 < L) {
+                const double vn = (A[l + 1] * g.z + C[l + 1]) * v - B[l + 1] * vp;
+                vp = v;
+                v = vn;
+                if (((l - M) & 7) == 7) {
+                    const double big = fabs(v) > fabs(vp) ? fabs(v) : fabs(vp);
+                    if (big > 0x1p256 || (big < 0x1p-256 && big > 0)) {
+                        int t;
+                        frexp(big, &t);
+                        v = ldexp(v, -t);
+                        vp = ldexp(vp, -t);
+                        e += t;
+                        scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
+                    }
+                }
+            }
+        }
+        out[4 * q + 0] = dre;
+        out[4 * q + 1] = dim;
+        out[4 * qm + 2] = mre;
+        out[4 * qm + 3] = mim;
+    }
+    free(A);
+}
+
 /* One order m: acc[4 l + (dre, dim, mre, mim)] = sum_r lambda_lm(theta_r) (direct, mirror),
  * the direct value of ring r at dbase + 2 r dstride, the mirror value of ring r (the
  * spectrum at -m of the ring reflected through the equator) at mbase + 2 r mstride. */
@@ -65,17 +143,7 @@ static void ana_one(int nside, int lmax, int s, int m, const double* dbase, int6
     double* A = (double*)malloc(sizeof(double) * 3 * L);
     double* C = A + L;
     double* B = A + 2 * L;
-    for (int lp = M + 1; lp < L; ++lp) {
-        const int l = lp - 1;
-        const double K = (l + 1.0) * (2.0 * l + 1.0) /
-                         sqrt(((l + 1.0) * (l + 1.0) - (double)m * m) * ((l + 1.0) * (l + 1.0) - (double)s * s));
-        A[lp] = sqrt((2.0 * l + 3.0) / (2.0 * l + 1.0)) * K;
-        C[lp] = l > 0 ? A[lp] * m * s / ((double)l * (l + 1.0)) : 0.0;
-        B[lp] = l > 0 ? sqrt((2.0 * l + 3.0) / (2.0 * l - 1.0)) * K *
-                            sqrt(((double)l * l - (double)m * m) * ((double)l * l - (double)s * s)) /
-                            ((double)l * (2.0 * l + 1.0))
-                      : 0.0;
-    }
+    rec_coefficients(L, m, s, A, C, B);
     const double c0 = seed_log2_const(m, s);
     const double sign = ((m + s) & 1) ? -1.0 : 1.0;
     for (int r = 0; r < nring; ++r) {
@@ -139,6 +207,18 @@ void gmo_lat_ana_s_cols(int nside, int lmax, int spin, const double* dcol, const
     for (int io = 0; io < norders; ++io)
         ana_one(nside, lmax, spin, orders[io], dcol + 2 * io * nring, 1, mcol + 2 * io * nring, 1,
                 out + (int64_t)io * L * 4);
+}
+
+/* Synthesis per sampled order: cd[io][l], cm[io][l] complex (L per order) ->
+ * out[io][r][4] = (direct re, im, mirror re, im) over all 4 nside - 1 rings. */
+void gmo_lat_syn_s(int nside, int lmax, int spin, const double* cd, const double* cm, const int* orders,
+                   int norders, double* out) {
+    const int L = lmax + 1;
+    const int64_t nring = 4 * (int64_t)nside - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int io = 0; io < norders; ++io)
+        syn_one(nside, lmax, spin, orders[io], cd + 2 * (int64_t)io * L, cm + 2 * (int64_t)io * L,
+                out + (int64_t)io * nring * 4);
 }
 
 void gmo_coupling(int lmax, const double* W, int lmax_w, const double* g, const int* rows, int nrows,

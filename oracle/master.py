@@ -3,7 +3,8 @@
 Restates ``gmaster/workspaces.py:86-95`` (coupled C_ell), ``gmaster/bins.py:30-37,
 65-101, 228-250`` (linear bands, weights, ``bin_cell``), ``gmaster/workspaces.py:
 1452-1470, 1978-2003`` (binning operators, binned MCM) and ``:2106-2147``
-(``decouple_cell``).  Pinned by the direct definitions (explicit loops over m and
+(``decouple_cell``), ``:2074-2104`` (``couple_cell``) and ``:1628-1634, 1970-1990,
+2149-2160`` (bandpower windows).  Pinned by the direct definitions (explicit loops over m and
 bands in ``tests/test_oracle_master.py``) and, end to end, by the full-sky identity
 (decoupled bandpowers == binned input pseudo-C_ell when the mask is 1).
 """
@@ -70,6 +71,31 @@ def decouple(mcm_binned, cl_coupled, lmax, ells, bpws, weights, ncls):
     nb = bpws.max() + 1
     binned = bin_cell(cl_coupled, lmax, ells, bpws, weights).T.reshape(-1)
     return np.linalg.solve(mcm_binned, binned).reshape((nb, ncls)).T
+
+
+def _beam_diag(beam, ncls, lmax):
+    b = np.ones(lmax + 1) if beam is None else np.asarray(beam, dtype=np.float64)[: lmax + 1]
+    return np.repeat(b, ncls)  # MCM index l * ncls + c
+
+
+def couple_cell(mcm, cl_in, ncls, lmax, beam=None):
+    """Expected coupled pseudo-spectra of full-sky spectra ``cl_in`` (``(ncls, >= L)``):
+    ``(M diag(b1 b2) C)`` in the ``l * ncls + c`` ordering, back as ``(ncls, L)``
+    (``gmaster/workspaces.py:2074-2104``)."""
+    L = lmax + 1
+    flat = np.asarray(cl_in, dtype=np.float64)[:, :L].T.reshape(-1) * _beam_diag(beam, ncls, lmax)
+    return (mcm @ flat).reshape(L, ncls).T
+
+
+def bandpower_windows(mcm, ncls, lmax, out_op, th_op, beam=None):
+    """``(ncls, nb, ncls, L)`` windows ``F`` with ``<C_b^c> = sum_{c', l} F[c, b, c', l] C^{c'}_l``:
+    the binned MCM ``W M diag(b1 b2) T`` solved against the one-sided ``W M diag(b1 b2)``
+    (``gmaster/workspaces.py:1628-1634, 1970-1990, 2149-2160``)."""
+    eye = np.eye(ncls)
+    one_sided = np.kron(out_op, eye) @ mcm * _beam_diag(beam, ncls, lmax)[None, :]
+    binned = one_sided @ np.kron(th_op, eye)
+    nb = out_op.shape[0]
+    return np.linalg.solve(binned, one_sided).reshape(nb, ncls, lmax + 1, ncls).transpose(1, 0, 3, 2)
 
 
 def workspace(pcl_mask, lmax, spin1, spin2, nlb):

@@ -57,6 +57,19 @@ def test_nchunks():
     assert _lib.query("gm_lat_nchunks", 8192, 0) == 4
 
 
+@pytest.fixture(scope="module")
+def dummy_pointer():
+    import torch
+
+    if not torch.cuda.is_available():
+        yield 1  # never dereferenced: no launch can happen without a device
+        return
+    buf = torch.zeros(1 << 23, dtype=torch.float64, device="cuda")  # 64 MiB
+    yield buf.data_ptr()
+    torch.cuda.synchronize()
+    del buf
+
+
 @pytest.mark.parametrize(
     "name,args,code",
     [
@@ -90,8 +103,10 @@ def test_nchunks():
         ("gm_gauss_draws", (1, 0, 1, 1, None), 1),                          # no elements
     ],
 )
-def test_argument_validation(name, args, code):
-    conv = [ctypes.c_void_p(a) if (a is not None and isinstance(a, int) and t is ctypes.c_void_p) else a
+def test_argument_validation(name, args, code, dummy_pointer):
+    # pointer arguments that are not under test point at a real (zeroed) device buffer when a
+    # GPU is visible, so a regressed check launches on valid memory instead of address 0x1
+    conv = [ctypes.c_void_p(dummy_pointer) if (a is not None and isinstance(a, int) and t is ctypes.c_void_p) else a
             for a, t in zip(args, _lib.SIGNATURES[name])]
     lib = _lib.load()
     got = getattr(lib, name)(*conv)

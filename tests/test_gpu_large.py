@@ -1,6 +1,7 @@
 """The Legendre kernels in their large-geometry configurations (4-wave workgroups, 5-8
-rings per lane, several ring chunks) against the C/OpenMP oracle on a sample of orders,
-FP64 and mixed.  These are the configurations the Nside-4096 benchmark runs; the numpy
+rings per lane, several ring chunks; syntheses: 2-wave workgroups up to 2048 rings, 4-wave
+above) against the C/OpenMP oracle on a sample of orders, FP64 and mixed, analyses and
+syntheses.  These are the configurations the Nside-4096 benchmark runs; the numpy
 oracle is too slow here, the C oracle (an independent natural-form recurrence with its own
 extended exponent, itself pinned against the numpy oracle) is not."""
 
@@ -107,6 +108,100 @@ def test_spin2_analysis_large_geometry(cb, nside, mixed):
         for c in range(2):
             e = _rel(alm[c][idx], eb[c][idx])
             assert e < (2e-6 if mixed else 1e-10), (c, m, e)
+    print("  ok", flush=True)
+
+
+def _gather_orders(alm, lmax, orders):
+    """``(ncomp, norders, L)`` coefficients a_lm (zero below l = m) of the sampled orders,
+    gathered on the device."""
+    L = lmax + 1
+    out = torch.zeros((alm.shape[0], len(orders), L), dtype=torch.complex128, device=alm.device)
+    for k, m in enumerate(orders):
+        start = m * (2 * lmax + 1 - m) // 2
+        out[:, k, m:] = alm[:, start + m:start + L]
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("nside", [1024, 4096])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_spin0_synthesis_large_geometry(cb, nside, mixed):
+    """Spin-0 synthesis of a map pair (the pipeline's field + mask; a pair takes the mixed
+    kernel) at Nside 1024 and 4096 (2048 and 8192 folded rings: the 4-wave configurations
+    with 4 (FP64) and 5 (mixed) rings per lane) against the C oracle's direct channel."""
+    from gmaster_amd import sht
+
+    print(f"spin-0 synthesis large geometry nside={nside} mixed={mixed}", flush=True)
+    lmax = 3 * nside - 1
+    L = lmax + 1
+    nring = 4 * nside - 1
+    nelem = L * (L + 1) // 2
+    g = torch.Generator("cuda").manual_seed(nside + 7)
+    alm = torch.complex(torch.randn((2, nelem), dtype=torch.float64, device="cuda", generator=g),
+                        torch.randn((2, nelem), dtype=torch.float64, device="cuda", generator=g))
+    alm[:, :L] = alm[:, :L].real.to(torch.complex128)  # m = 0 real
+    p = sht.plan(nside, lmax, 0, "cuda")
+    spec = torch.empty((2, nring, L), dtype=torch.complex128, device="cuda")
+    sht._synthesis_s0(p, alm, spec, mixed)
+    orders = _orders(L)
+    got = spec[:, :, torch.as_tensor(orders, device="cuda")].cpu().numpy()  # (2, nring, norders)
+    coef = _gather_orders(alm, lmax, orders)
+    del spec
+    for k in range(2):
+        out = cb.lat_syn_s(nside, lmax, 0, coef[k], np.zeros_like(coef[k]), orders)
+        for n, m in enumerate(orders):
+            ref = out[n, :, 0] + 1j * out[n, :, 1]
+            e = _rel(got[k, :, n], ref)
+            assert e < (2e-6 if mixed else 1e-10), (k, m, e)
+    print("  ok", flush=True)
+
+
+@pytest.mark.parametrize("nside", [512, 1024, 4096])
+@pytest.mark.parametrize("mixed", [False, True])
+def test_spin2_synthesis_large_geometry(cb, nside, mixed):
+    """Spin-2 synthesis at Nside 512 (2047 rings: 2-wave configuration), 1024 and 4096
+    (4095 / 16383 rings: the 4-wave R = 6 configurations of the benchmark) against the C
+    oracle's direct and mirror channels; the E/B -> a+ combination restates
+    ``gmaster/_sht/healpix.py:599-611``."""
+    from gmaster_amd import sht
+
+    print(f"spin-2 synthesis large geometry nside={nside} mixed={mixed}", flush=True)
+    s = 2
+    lmax = 3 * nside - 1
+    L = lmax + 1
+    W = 2 * L - 1
+    nring = 4 * nside - 1
+    nelem = L * (L + 1) // 2
+    g = torch.Generator("cuda").manual_seed(nside + 9)
+    eb = torch.complex(torch.randn((2, nelem), dtype=torch.float64, device="cuda", generator=g),
+                       torch.randn((2, nelem), dtype=torch.float64, device="cuda", generator=g))
+    eb[:, :L] = eb[:, :L].real.to(torch.complex128)
+    p = sht.plan(nside, lmax, s, "cuda")
+    spec = torch.empty((nring, W), dtype=torch.complex128, device="cuda")
+    sht._synthesis_s(p, eb[0], eb[1], spec, mixed)
+    orders = _orders(L)
+    cols = torch.as_tensor(np.concatenate([L - 1 + orders, L - 1 - orders]), device="cuda")
+    got = spec[:, cols].cpu().numpy()
+    del spec
+    E, B = _gather_orders(eb, lmax, orders)
+    ell = np.arange(L)
+    sgn_m = ((-1.0) ** orders)[:, None]
+    plus = -(E + 1j * B)                                # a+_{l, m}
+    minus = -sgn_m * (np.conj(E) + 1j * np.conj(B))     # a+_{l, -m}
+    for n, m in enumerate(orders):
+        plus[n, :max(m, s)] = 0
+        minus[n, :max(m, s)] = 0
+    sg = (-1.0) ** s
+    cd = sg * plus
+    cm = sg * ((-1.0) ** (ell + s))[None, :] * minus
+    out = cb.lat_syn_s(nside, lmax, s, cd, cm, orders)
+    no = len(orders)
+    for n, m in enumerate(orders):
+        scale = max(np.max(np.abs(out[n, :, 0:2])), np.max(np.abs(out[n, :, 2:4])))
+        ed = np.max(np.abs(got[:, n] - (out[n, :, 0] + 1j * out[n, :, 1]))) / scale
+        assert ed < (2e-6 if mixed else 1e-10), ("direct", m, ed)
+        if m:
+            em = np.max(np.abs(got[:, no + n] - (out[n, :, 2] + 1j * out[n, :, 3]))) / scale
+            assert em < (2e-6 if mixed else 1e-10), ("mirror", m, em)
     print("  ok", flush=True)
 
 

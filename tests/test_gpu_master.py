@@ -306,3 +306,40 @@ def test_teb_workspace_matches_oracle_layout_and_separate_workspaces(tmp_path):
     r = nmt.NmtWorkspace.from_file(path)
     assert r.is_teb and r.ncls == 7
     np.testing.assert_allclose(np.asarray(r.decouple_cell(cl)), bp, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("nside,nlb,fname", [(16, 4, "pipeline_nside16.npz"), (64, 16, "pipeline_nside64.npz")])
+@pytest.mark.parametrize("spins", [(0, 0), (0, 2), (2, 2)])
+def test_bandpower_windows_and_couple_cell_match_oracle(nside, nlb, fname, spins):
+    """``get_bandpower_windows`` and ``couple_cell`` (with beams) against
+    ``oracle.master.bandpower_windows`` / ``couple_cell`` built from the fixture's mask
+    pseudo-C_ell (the oracle's windows are pinned in tests/test_oracle_master.py)."""
+    import gmaster as nmt
+    from oracle.coupling import assemble_mcm
+
+    fx = np.load(os.path.join(GOLDEN, fname))
+    mask = torch.as_tensor(fx["mask"], device="cuda")
+    tqu = torch.as_tensor(fx["tqu"], device="cuda")
+    fields = {0: nmt.NmtField(mask, tqu[0:1], n_iter=3), 2: nmt.NmtField(mask, tqu[1:3], n_iter=3)}
+    lmax = 3 * nside - 1
+    b = nmt.NmtBin.from_nside_linear(nside, nlb)
+    w = nmt.NmtWorkspace.from_fields(fields[spins[0]], fields[spins[1]], b)
+    beam1, beam2 = np.linspace(1, 0.5, lmax + 1), np.linspace(1, 0.8, lmax + 1)
+    w.update_beams(beam1, beam2)
+    mcm, ncls = assemble_mcm(oracle_blocks(fx["pcl_mask"], lmax), *spins)
+    ells, bpws, wts = omaster.linear_bins(lmax, nlb)
+    out_op, th_op = omaster.binning_operators(lmax, ells, bpws, wts)
+    ref = omaster.bandpower_windows(mcm, ncls, lmax, out_op, th_op, beam1 * beam2)
+    got = np.asarray(w.get_bandpower_windows())
+    err = _rel(got, ref)
+    print(f"nside={nside} spins={spins} windows rel={err:.2e}", flush=True)
+    assert got.shape == ref.shape and err < 1e-9
+    rng = np.random.default_rng(nside + spins[1])
+    cl = rng.normal(size=(ncls, lmax + 3))
+    cc = np.asarray(w.couple_cell(cl))
+    err_c = _rel(cc, omaster.couple_cell(mcm, cl, ncls, lmax, beam1 * beam2))
+    print(f"  couple_cell rel={err_c:.2e}", flush=True)
+    assert err_c < 1e-9
+    # the device windows reproduce the device decoupling of the coupled expectation
+    bp = np.asarray(w.decouple_cell(w.couple_cell(cl)))
+    assert _rel(np.einsum("cbdl,dl->cb", got, cl[:, :lmax + 1]), bp) < 1e-10

@@ -133,11 +133,14 @@ def test_fast_coupling_spot_check_falls_back_when_tolerance_is_tiny():
 
 @pytest.mark.parametrize("nside,spin", [(512, 0), (256, 2), (512, 2)])
 def test_large_geometry_round_trip(nside, spin):
-    """The large-geometry kernel configurations (many rings per lane, 4-wave workgroups, a
-    different ring chunking in each precision), which the numpy oracle is too slow to reach:
-    band-limited alms (l <= 2 nside) come back from synthesis + analysis with 3 Jacobi steps
-    to the HEALPix quadrature's accuracy (~1e-4 here; an indexing or chunking mistake gives
-    O(1)), and the FP64 and mixed configurations agree to FP32 rounding."""
+    """Mid-size geometries, which the numpy oracle is too slow to reach: band-limited alms
+    (l <= 2 nside) come back from synthesis + analysis with 3 Jacobi steps to the HEALPix
+    quadrature's accuracy (~1e-4 here; an indexing or chunking mistake gives O(1)), and the
+    FP64 and mixed configurations agree to FP32 rounding.  Configurations reached:
+    (512, 0) analysis 2-wave R 4 (1024 folded rings <= 2048), synthesis 4-wave (large R);
+    (256, 2) and (512, 2) analysis 4-wave (large R, 1023 / 2047 rings > 512), synthesis
+    2-wave R 4 (<= 2048 rings).  The 4-wave spin-s syntheses and the large analyses at the
+    benchmark's size are checked against the C oracle in tests/test_gpu_large.py."""
     from gmaster_amd import sht
 
     print(f"round trip nside={nside} spin={spin}", flush=True)
