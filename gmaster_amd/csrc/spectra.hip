@@ -1,5 +1,7 @@
 // Power spectra and binning on gfx950, FP64: coupled pseudo-C_ell of healpy-packed alms
 // (HBM-bound segmented reduction over m), binned spectra and binned coupling blocks.
+#include <mutex>
+
 #include "common.h"
 
 namespace gm {
@@ -95,6 +97,60 @@ extern "C" int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm
                        (int)n2, alm_stride, (int)L, work);
     hipLaunchKernelGGL(k_cl_final, dim3((unsigned)((L + 255) / 256), np), dim3(256), 0, st, work, np, (int)L, cl);
     return check_launch("gm_coupled_cell");
+}
+
+// ------------------------------------------------------------------ no-allocation variant
+// The partial sums of gm_coupled_cell live in a file-scope device array instead of a
+// caller's workspace.  Each GPU holds its own copy for the life of the process; host code
+// uses the per-device address returned by hipGetSymbolAddress, never the symbol itself.
+namespace gm {
+namespace {
+constexpr int64_t kClScratchDoubles = int64_t(1) << 23;  // 64 MiB: lmax <= 8191 for 4 pairs
+__device__ __align__(256) double scratch[kClScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of the current device's scratch array (resolved once per device).
+int scratch_for_current_device(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(GM_E_LAUNCH, "gm_coupled_cell_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(GM_E_ARG, "gm_coupled_cell_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GM_E_LAUNCH, "gm_coupled_cell_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return GM_OK;
+}
+}  // namespace
+}  // namespace gm
+
+extern "C" int64_t gm_coupled_cell_noalloc_max_work(void) { return kClScratchDoubles; }
+
+extern "C" int gm_coupled_cell_noalloc(const double* alm1, int64_t n1, const double* alm2, int64_t n2,
+                                       int64_t alm_stride, int64_t lmax, double* cl, void* stream) {
+    if (!alm1 || !alm2 || !cl) return fail(GM_E_ARG, "gm_coupled_cell_noalloc: null pointer");
+    if (lmax < 0) return fail(GM_E_ARG, "gm_coupled_cell_noalloc: lmax < 0");
+    if (n1 < 1 || n1 > 2 || n2 < 1 || n2 > 2)
+        return fail(GM_E_ARG, "gm_coupled_cell_noalloc: 1 or 2 components per field");
+    const int64_t need = gm_cl_work_size(lmax, n1 * n2);
+    if (need > kClScratchDoubles)
+        return fail(GM_E_SHAPE,
+                    "gm_coupled_cell_noalloc: lmax %lld with %lld pairs needs %lld scratch doubles, the maximum is "
+                    "%lld (use gm_coupled_cell with a workspace)",
+                    (long long)lmax, (long long)(n1 * n2), (long long)need, (long long)kClScratchDoubles);
+    double* work = nullptr;
+    const int rc = scratch_for_current_device(&work);
+    if (rc != GM_OK) return rc;
+    // every partial the final pass reads is written by the partial pass: nothing to clear
+    return gm_coupled_cell(alm1, n1, alm2, n2, alm_stride, lmax, cl, work, stream);
 }
 
 extern "C" int gm_bin_cell(const double* cl, int64_t ncl, int64_t ld, const int32_t* ells, const int32_t* band_start,

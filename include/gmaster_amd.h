@@ -158,6 +158,17 @@ int64_t gm_cl_work_size(int64_t lmax, int64_t npairs);
 int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t n2, int64_t alm_stride,
                     int64_t lmax, double* cl, double* work, void* stream);
 
+/* gm_coupled_cell without a caller workspace and without any allocation: the partial sums
+ * use a 64 MiB device array of the library (gm_coupled_cell_noalloc_max_work() doubles per
+ * GPU, held until the process ends), so lmax <= 8191 with 2 x 2 components (lmax <= 16383
+ * with 1 x 1); larger requests fail with GM_E_SHAPE.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (graph capture, tight serving loops).  All
+ * calls on one GPU share its array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller. */
+int64_t gm_coupled_cell_noalloc_max_work(void);
+int gm_coupled_cell_noalloc(const double* alm1, int64_t n1, const double* alm2, int64_t n2, int64_t alm_stride,
+                            int64_t lmax, double* cl, void* stream);
+
 /* Mode-coupling blocks 0-0, 0-2, 2-2 even / odd in one pass over the Wigner-3j rows:
  *   M[l1][l2] = (2 l2 + 1) sum_l3 W'_l3 (l1 l2 l3; ...)(l1 l2 l3; ...)
  * window: the weighted mask pseudo-C_ell W'_l = (2l+1) W_l / (4 pi), lmax_window+1 doubles

@@ -32,6 +32,14 @@ def test_abi_version():
     assert _lib.query("gm_abi_version") == 2
 
 
+def test_noalloc_scratch_limit():
+    """The no-allocation coupled-C_ell entry holds 2^23 doubles (64 MiB) of partial sums:
+    enough for lmax 8191 with 2 x 2 components, not for lmax 8192."""
+    assert _lib.query("gm_coupled_cell_noalloc_max_work") == 1 << 23
+    assert _lib.query("gm_cl_work_size", 8191, 4) <= 1 << 23 < _lib.query("gm_cl_work_size", 8192, 4)
+    assert _lib.query("gm_cl_work_size", 16383, 1) <= 1 << 23
+
+
 def test_table_size_matches_layout():
     for lmax in (0, 5, 191, 12287):
         L = lmax + 1
@@ -86,6 +94,8 @@ def dummy_pointer():
         ("gm_coupling", (None, 10, 1, 10, 1, None, None, None, None), 1),   # null window
         ("gm_coupling", (1, 21, 1, 10, 1, None, None, None, None), 1),      # window beyond 2 lmax
         ("gm_coupled_cell", (1, 3, 1, 1, 66, 10, 1, 1, None), 1),           # 3 components
+        ("gm_coupled_cell_noalloc", (None, 1, 1, 1, 66, 10, 1, None), 1),   # null alm
+        ("gm_coupled_cell_noalloc", (1, 2, 1, 2, 1 << 26, 9000, 1, None), 2),  # partials beyond the 64 MiB array
         ("gm_coupling_pairs", (1, 20, 1, 10, None, None, 5, 1, None, None, None, None), 1),  # null pair list
         ("gm_coupling_pairs", (1, 20, 1, 10, 1, 1, -1, 1, None, None, None, None), 1),      # negative count
         ("gm_toeplitz", (1, 10, 10, 3, 2, 2, None), 1),                     # l_toeplitz = lmax
