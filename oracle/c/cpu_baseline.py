@@ -37,6 +37,8 @@ def load():
     lib.gmo_lat_ana_s_cols.restype = None
     lib.gmo_lat_syn_s.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, p, p, p, ctypes.c_int, p]
     lib.gmo_lat_syn_s.restype = None
+    lib.gmo_lat_syn_s_ext.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, p, p, p, ctypes.c_int, p]
+    lib.gmo_lat_syn_s_ext.restype = None
     lib.gmo_coupling.argtypes = [ctypes.c_int, p, ctypes.c_int, p, p, ctypes.c_int, p]
     lib.gmo_coupling.restype = None
     return lib
@@ -70,10 +72,12 @@ def lat_ana_s_cols(nside, lmax, spin, dcol, mcol, orders):
     return out
 
 
-def lat_syn_s(nside, lmax, spin, cd, cm, orders):
+def lat_syn_s(nside, lmax, spin, cd, cm, orders, extended=False):
     """``(norders, nring, 4)``: per ring, direct ``sum_l cd[k][l] lambda_lm(theta_r)`` and
     mirror ``sum_l cm[k][l] lambda_lm(pi - theta_r)`` (re, im); ``cd``, ``cm`` are
-    ``(len(orders), L)`` complex."""
+    ``(len(orders), L)`` complex.  ``extended=True`` runs the recurrence and sums in x87
+    extended precision (a reference for judging FP64 results at the polar rings, where
+    the FP64 recurrence's own rounding reaches ~1e-10 relative at L = 12288)."""
     lib = load()
     L = lmax + 1
     cd = np.ascontiguousarray(cd, dtype=np.complex128)
@@ -82,8 +86,8 @@ def lat_syn_s(nside, lmax, spin, cd, cm, orders):
     if cd.shape != (len(orders), L) or cm.shape != cd.shape:
         raise ValueError("coefficients must be (len(orders), lmax + 1)")
     out = np.zeros((len(orders), 4 * nside - 1, 4))
-    lib.gmo_lat_syn_s(nside, lmax, spin, cd.ctypes.data, cm.ctypes.data, orders.ctypes.data, len(orders),
-                      out.ctypes.data)
+    fn = lib.gmo_lat_syn_s_ext if extended else lib.gmo_lat_syn_s
+    fn(nside, lmax, spin, cd.ctypes.data, cm.ctypes.data, orders.ctypes.data, len(orders), out.ctypes.data)
     return out
 
 

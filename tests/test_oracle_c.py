@@ -80,10 +80,12 @@ def test_column_entry_matches_full_spectrum_entry():
     assert np.array_equal(full, cols)
 
 
+@pytest.mark.parametrize("extended", [False, True])
 @pytest.mark.parametrize("spin", [0, 1, 2])
-def test_c_legendre_synthesis_matches_numpy_oracle(cb, spin):
-    """``gmo_lat_syn_s`` (direct and mirror channels) against ``synthesis_plus`` /
-    ``synthesis_s0`` of the numpy oracle, every order of a small geometry."""
+def test_c_legendre_synthesis_matches_numpy_oracle(cb, spin, extended):
+    """``gmo_lat_syn_s`` (direct and mirror channels; FP64 and extended precision) against
+    ``synthesis_plus`` / ``synthesis_s0`` of the numpy oracle, every order of a small
+    geometry."""
     nside, lmax = 8, 23
     L = lmax + 1
     R = Rings(nside)
@@ -99,7 +101,7 @@ def test_c_legendre_synthesis_matches_numpy_oracle(cb, spin):
     sg = (-1.0) ** spin
     cd = sg * ap[:, L - 1 + orders].T
     cm = sg * ((-1.0) ** (ell + spin))[None, :] * ap[:, L - 1 - orders].T
-    out = cb.lat_syn_s(nside, lmax, spin, cd, cm, orders)
+    out = cb.lat_syn_s(nside, lmax, spin, cd, cm, orders, extended=extended)
     if spin == 0:
         alm = np.zeros(ai.nelem, dtype=np.complex128)
         for m in range(L):
@@ -114,6 +116,28 @@ def test_c_legendre_synthesis_matches_numpy_oracle(cb, spin):
         assert np.max(np.abs(out[m, :, 0] + 1j * out[m, :, 1] - F[:, L - 1 + m])) < 1e-12 * scale
         if m:
             assert np.max(np.abs(out[m, :, 2] + 1j * out[m, :, 3] - F[:, L - 1 - m])) < 1e-12 * scale
+
+
+def test_c_extended_synthesis_bounds_fp64_at_the_pole(cb):
+    """At Nside 4096, m = 0, the FP64 and extended-precision syntheses agree to ~1e-10 of the
+    largest value, with the difference largest at the polar rings (the FP64 recurrence's
+    conditioning as x = cos(theta) -> 1) and ~1e-12 in the belt."""
+    nside = 4096
+    lmax = 3 * nside - 1
+    L = lmax + 1
+    rng = np.random.default_rng(4)
+    orders = np.array([0])
+    cd = rng.normal(size=(1, L)) + 1j * rng.normal(size=(1, L))
+    cd[:, 0] = cd[:, 0].real
+    cm = np.zeros_like(cd)
+    a = cb.lat_syn_s(nside, lmax, 0, cd, cm, orders)
+    b = cb.lat_syn_s(nside, lmax, 0, cd, cm, orders, extended=True)
+    d = np.abs((a[0, :, 0] - b[0, :, 0]) + 1j * (a[0, :, 1] - b[0, :, 1]))
+    scale = np.max(np.abs(b[0, :, 0] + 1j * b[0, :, 1]))
+    assert d.max() / scale < 1e-9
+    belt = slice(nside, 3 * nside)
+    assert d[belt].max() / scale < 1e-11
+    assert d.max() > 10 * d[belt].max()
 
 
 def test_c_synthesis_is_the_analysis_transpose(cb):
