@@ -57,10 +57,9 @@ def legendre_flops(nside, lmax):
     return 2 * (2 + 4) * T0, 2 * (2 + 4) * T2
 
 
-def _measured_traffic(nside, mixed=False):
-    """HBM bytes per k_ana_s launch from the committed rocprofv3 FETCH_SIZE + WRITE_SIZE
-    summary at this Nside (copy-kernel scaled), or None when there is none."""
-    name = "k_ana_s_mixed" if mixed else "k_ana_s"
+def _measured_traffic(nside, name):
+    """HBM bytes per launch of kernel ``name`` from the committed rocprofv3 FETCH_SIZE +
+    WRITE_SIZE summary at this Nside (copy-kernel scaled), or None when there is none."""
     path = os.path.join(ROOT, "profiles", f"{name}_traffic_nside{nside}.txt")
     if not os.path.exists(path):
         return None
@@ -71,10 +70,11 @@ def _measured_traffic(nside, mixed=False):
 
 
 def run_pipeline(nmt, mask, tqu, nside, nlb, form="fp64", info=None):
-    from gmaster_amd import field, sht, workspaces
+    from gmaster_amd import batched, field, sht, workspaces
 
     nmt.set_form(form)
     sht.clear_plans()
+    batched.clear_plans()
     workspaces.clear_coupling_cache()
     field.clear_mask_cache()
     fT = nmt.NmtField(mask, tqu[0:1], n_iter=3)
@@ -307,6 +307,31 @@ def main():
     fp64_s = sum(times_fp64) / len(times_fp64)
     f0, f2 = legendre_flops(nside, lmax)
 
+    def roof_batched(st):
+        # the batched analysis: one spin-0 recurrence (2 FP64 FMA per (l, m, ring pair))
+        # shared by 12 real channels (T, mask, Q, U, Q/sin^2, U/sin^2: 12 FP32 FMA on the
+        # matrix unit), i.e. 2 (2 + 2k) T0 FLOP with k = 6 complex channels
+        dom_ms = st.get("gm_bat_ana", {}).get("mean_ms", float("nan"))
+        L = lmax + 1
+        T0 = L * (L + 1) / 2 * 2 * nside
+        flop = 2 * (2 + 12) * T0
+        achieved = flop / (dom_ms * 1e-3)
+        peak = 28.0 / (4.0 / PEAK_FP64 + 24.0 / PEAK_FP32)
+        return {
+            "kernel": "k_bat_ana (T + mask + Q/U analysis on one recurrence, gm_bat_ana: FP32 ring blocks on the "
+                      "matrix unit + FP64 polar blocks)",
+            "bound": "fp64 valu + fp32 mfma",
+            "pipe": "FP64 VALU recurrence (78.6 TFLOP/s) + FP32 MFMA contraction (157.3 TFLOP/s): op-mix peak",
+            "achieved": achieved / 1e12,
+            "peak": peak / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": _measured_traffic(nside, "k_bat_ana"),
+            "flop_per_launch": flop,
+            "mean_launch_ms": dom_ms,
+            "replaces_ms": None,
+        }
+
     def roof(st, mixed):
         name = "gm_lat_ana_s_mixed" if mixed else "gm_lat_ana_s"
         dom_ms = st.get(name, {}).get("mean_ms", float("nan"))
@@ -316,19 +341,22 @@ def main():
         peak = PEAK_FP64 if not mixed else 12.0 / (4.0 / PEAK_FP64 + 8.0 / PEAK_FP32)
         return {
             "kernel": f"k_ana_s{'<mixed>' if mixed else ''} (spin-2 Legendre analysis, {name})",
-            "bound": "mfma",
+            "bound": "valu",
             "pipe": ("FP64 VALU (MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s)" if not mixed else
                      "FP64 VALU recurrence (78.6 TFLOP/s) + FP32 VALU contraction (157.3 TFLOP/s): op-mix peak"),
             "achieved": achieved / 1e12,
             "peak": peak / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": _measured_traffic(nside, mixed),
+            "traffic": _measured_traffic(nside, "k_ana_s_mixed" if mixed else "k_ana_s"),
             "flop_per_launch": f2,
             "mean_launch_ms": dom_ms,
         }
 
-    roofline = roof(stages, args.form != "fp64")
+    if "gm_bat_ana" in stages:
+        roofline = roof_batched(stages)
+    else:
+        roofline = roof(stages, args.form != "fp64")
     roofline_fp64 = roof(stages_fp64, False)
     ana0_ms = stages_fp64.get("gm_lat_ana_s0", {}).get("mean_ms", float("nan"))
     cpl_ms = stages_fp64.get("gm_coupling", {}).get("mean_ms", float("nan"))
@@ -354,7 +382,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": ("f64" if args.form == "fp64" else
+                  "f64 recurrence, f32 contraction (f64 within 14.5 deg of the poles), f64 coupling and LU"),
         "data": "synthetic",
         "form": args.form,
         "fp64_s": fp64_s,

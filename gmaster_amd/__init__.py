@@ -9,10 +9,10 @@ Every transform, coupling matrix and spectrum runs in the HIP library
 ``libgmaster_amd.so`` (built by ``python -m gmaster_amd.build``); there is no CPU path.
 """
 
-from . import _lib, sht  # noqa: F401
+from . import _lib, batched, sht  # noqa: F401
 from ._array import GmArray
-from ._config import (get_default_params, nmt_params, set_form, set_n_iter_default, set_sht_calculator,
-                      set_tol_pinv_default)
+from ._config import (get_default_params, nmt_params, set_batching, set_form, set_n_iter_default,
+                      set_sht_calculator, set_tol_pinv_default)
 from .bins import NmtBin
 from .deproject import compute_full_master, deprojection_bias, moore_penrose_pinvh, uncorr_noise_deprojection_bias
 from .field import NmtField
@@ -49,6 +49,7 @@ __all__ = [
     "mask_apodization",
     "moore_penrose_pinvh",
     "nmt_params",
+    "set_batching",
     "set_form",
     "set_n_iter_default",
     "set_sht_calculator",

@@ -12,6 +12,9 @@ class NmtParams:
         self.form = "fp64"
         self.fast_toeplitz = (2000, 256, 256)
         self.fast_tol = 1e-3
+        # fast form: fields defer their transforms and run them batched (T, mask and Q/U
+        # of one geometry through one recurrence) when their alms are first needed
+        self.batch_fields = True
 
 
 nmt_params = NmtParams()
@@ -63,6 +66,11 @@ def set_form(form, toeplitz=None, tol=None):
         nmt_params.fast_tol = float(tol)
 
 
+def set_batching(on):
+    """Enable or disable deferred, batched field transforms in the fast form."""
+    nmt_params.batch_fields = bool(on)
+
+
 def get_default_params():
     return {
         "sht_calculator": nmt_params.sht_calculator,
@@ -70,4 +78,5 @@ def get_default_params():
         "n_iter_mask": nmt_params.n_iter_mask_default,
         "tol_pinv": nmt_params.tol_pinv_default,
         "form": nmt_params.form,
+        "batch_fields": nmt_params.batch_fields,
     }

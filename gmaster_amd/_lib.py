@@ -43,6 +43,12 @@ SIGNATURES = {
     "gm_lat_syn_s": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_lat_syn_s_mixed": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _i64, _p],
     "gm_coupled_cell": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _p],
+    "gm_bat_blocks": [_i64, _i64, _i64],
+    "gm_bat_stride": [_i64, _i64, _i64],
+    "gm_bat_ana": [_p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _i64, _i64, _p, _p, _p],
+    "gm_bat_finish": [_p, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
+    "gm_bat_prep": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p],
+    "gm_bat_syn": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p],
     "gm_coupled_cell_noalloc": [_p, _i64, _p, _i64, _i64, _i64, _p, _p],
     "gm_coupled_cell_noalloc_max_work": [],
     "gm_coupling": [_p, _i64, _p, _i64, _p, _p, _p, _p, _p],
@@ -80,7 +86,7 @@ SIGNATURES = {
     "gm_abi_version": [],
     "gm_last_error": [],
 }
-_RET_I64 = {"gm_coupled_cell_noalloc_max_work", "gm_apodize_work_size", "gm_lat_ring_chunks", "gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
+_RET_I64 = {"gm_bat_blocks", "gm_bat_stride", "gm_coupled_cell_noalloc_max_work", "gm_apodize_work_size", "gm_lat_ring_chunks", "gm_map_dots_work_size", "gm_lat_nchunks_mixed", "gm_table_size", "gm_lat_nchunks", "gm_cl_work_size"}
 
 _lib = None
 

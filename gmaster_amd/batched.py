@@ -1,0 +1,259 @@
+"""Batched transforms: spin-0 maps and spin-2 fields of one geometry through ONE spin-0
+recurrence per (m, ring), with the contraction on the matrix unit (``gm_bat_*``).
+
+A T+P realisation (T, its mask, Q/U) is 12 real channels: T and the mask contribute the
+ring-spectrum sums ``S0_l = sum_r lambda_l G_r``; Q and U contribute ``S0_l`` and
+``S1_l = sum_r lambda_l G_r / sin^2(theta_r)``, from which the finish kernel forms E and B
+with the HEALPix F1/F2 identities (the spin-2 rows as combinations of the spin-0 rows
+lambda_{l-2} .. lambda_{l+1}).  Rings farther than ~14.5 degrees from the poles contract
+in FP32 on the matrix unit over the FP64 recurrence; the polar ring pairs, where 1/sin^2
+grows and the spin-2 combinations cancel, contract in FP64.
+
+The Jacobi refinement (``gmaster/_sht/healpix.py:1459-1471``) runs on corrections as in
+``sht._refine``: after ``R_0 = fold(S alm_0) - G`` each pass analyses the residual and
+the next residual is updated by the synthesis of the correction, so the rounding of every
+pass after the first scales with the correction, not with the signal.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib, sht
+
+_PLANS = {}
+
+# Ring pairs with sin^2(theta) below this contract in FP64 (the 1/sin^2 channel's weight
+# stays <= 16 in FP32).
+POLAR_SIN2 = 1.0 / 16.0
+
+SUPPORTED = ((2, 1), (1, 1), (0, 1), (0, 2), (2, 0), (4, 0), (8, 0))
+
+# zero entries after the alpha table and spare degrees after the synthesis coefficients
+# (the kernels read whole 32-degree blocks past an order's last degree)
+ALPHA_TAIL = 64
+COEF_TAIL = 64
+
+
+@dataclass
+class BatPlan:
+    nside: int
+    lmax: int
+    device: torch.device
+    tab: sht.Plan          # spin-0 tables of lmax + 1 (alpha, P valid up to degree lmax + 1)
+    alpha: torch.Tensor    # tab.alpha followed by ALPHA_TAIL zeros (whole-block reads stay in bounds)
+    nact: torch.Tensor     # ring pairs visited by order m (prefix from the equator)
+    jsplit: int            # first ring pair contracted in FP64
+    mpol: int              # orders m < mpol reach beyond jsplit
+    nb32: int
+    nb64: int
+    tsize: int
+    tsize64: int           # table entries of the orders m < mpol
+
+
+def _nactive_pairs(nside, lmax, polar_skip):
+    """Ring pairs (equator first) that order m visits: the larger of ducc0's spin-0 and
+    spin-2 polar-skip bounds (``sht._nactive``), evaluated on the folded rings."""
+    z, sin = sht._ring_slots(nside, 0)
+    n = len(z)
+    L = lmax + 1
+    if not polar_skip:
+        return np.full(L, n, dtype=np.int32)
+    ofs = max(100.0, 0.01 * lmax)
+    b = lmax * sin + ofs
+    mlim0 = b
+    mlim2 = 2.0 * np.abs(z) + np.sqrt(np.maximum(b * b - 4.0 * sin * sin, 0.0))
+    best = np.maximum.accumulate(np.maximum(mlim0, mlim2)[::-1])[::-1]
+    return np.searchsorted(-best, -np.arange(L), side="right").astype(np.int32)
+
+
+def polar_split(nside):
+    """Number of ring pairs (from the equator) with sin^2(theta) >= POLAR_SIN2."""
+    _, sin = sht._ring_slots(nside, 0)
+    return int(np.count_nonzero(sin * sin >= POLAR_SIN2))
+
+
+def plan(nside, lmax, device=None, polar_skip=True):
+    device = torch.device(device if device is not None else "cuda")
+    if device.index is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    key = (int(nside), int(lmax), device, bool(polar_skip))
+    bp = _PLANS.get(key)
+    if bp is not None:
+        return bp
+    nside, lmax = int(nside), int(lmax)
+    tab = sht.plan(nside, lmax + 1, 0, device, polar_skip)
+    jsplit = polar_split(nside)
+    nact = _nactive_pairs(nside, lmax, polar_skip)
+    mpol = int(np.count_nonzero(nact > jsplit))
+    Lt = lmax + 2
+    alpha = torch.zeros(tab.alpha.numel() + ALPHA_TAIL, dtype=torch.float64, device=device)
+    alpha[: tab.alpha.numel()].copy_(tab.alpha)
+    bp = BatPlan(
+        nside=nside, lmax=lmax, device=device, tab=tab, alpha=alpha,
+        nact=torch.as_tensor(nact, dtype=torch.int32, device=device),
+        jsplit=jsplit, mpol=mpol, tsize64=mpol * (Lt + 8) - mpol * (mpol - 1) // 2,
+        nb32=int(_lib.query("gm_bat_blocks", nside, jsplit, 0)),
+        nb64=int(_lib.query("gm_bat_blocks", nside, jsplit, 1)),
+        tsize=int(_lib.query("gm_table_size", lmax + 1)),
+    )
+    _PLANS[key] = bp
+    return bp
+
+
+def clear_plans():
+    _PLANS.clear()
+
+
+def _check(k0, k2):
+    if (k0, k2) not in SUPPORTED:
+        raise ValueError(f"unsupported batch: {k0} spin-0 maps and {k2} spin-2 fields")
+
+
+def analysis(bp, spec0, spec2, alm0, alm2, mode):
+    """Spectra -> alms (mode 0 writes, 1 subtracts, 2 adds).  ``spec0`` ``(K0, nring, L)``,
+    ``spec2`` ``(K2, nring, 2L-1)`` (F+ of Q + iU), ``alm0`` ``(K0, nelem)``, ``alm2``
+    ``(K2, 2, nelem)``; absent kinds are None."""
+    k0 = 0 if spec0 is None else spec0.shape[0]
+    k2 = 0 if spec2 is None else spec2.shape[0]
+    _check(k0, k2)
+    st = _lib.stream_ptr(bp.device)
+    S = int(_lib.query("gm_bat_stride", k0, k2, 0))
+    part32 = torch.empty(bp.nb32 * bp.tsize * S, dtype=torch.float32, device=bp.device) if bp.nb32 else None
+    part64 = (torch.empty(bp.nb64 * bp.tsize64 * S, dtype=torch.float64, device=bp.device)
+              if bp.nb64 and bp.mpol else None)
+    t = bp.tab
+    _lib.call("gm_bat_ana", _lib.ptr(spec0), k0, spec0.shape[2] if k0 else 0, spec0.stride(0) if k0 else 0,
+              _lib.ptr(spec2), k2, spec2.shape[2] if k2 else 0, spec2.stride(0) if k2 else 0, bp.nside, bp.lmax,
+              _lib.ptr(bp.alpha), _lib.ptr(t.smant), _lib.ptr(t.sexp), _lib.ptr(bp.nact), bp.jsplit, bp.mpol,
+              _lib.ptr(part32), _lib.ptr(part64), st)
+    _lib.call("gm_bat_finish", _lib.ptr(part32), _lib.ptr(part64), k0, k2, bp.nside, bp.lmax, _lib.ptr(bp.nact),
+              bp.jsplit, bp.mpol, _lib.ptr(t.P), _lib.ptr(alm0), alm0.stride(0) if k0 else 0, _lib.ptr(alm2),
+              alm2.stride(1) if k2 else 0, int(mode), st)
+
+
+def synthesis(bp, alm0, alm2, spec0, spec2):
+    """Alms -> ring spectra (the inverse of ``analysis``'s layouts)."""
+    k0 = 0 if alm0 is None else alm0.shape[0]
+    k2 = 0 if alm2 is None else alm2.shape[0]
+    _check(k0, k2)
+    st = _lib.stream_ptr(bp.device)
+    NS = int(_lib.query("gm_bat_stride", k0, k2, 1))
+    coef = torch.empty((bp.tsize + COEF_TAIL) * NS, dtype=torch.float64, device=bp.device)
+    t = bp.tab
+    _lib.call("gm_bat_prep", _lib.ptr(alm0), k0, alm0.stride(0) if k0 else 0, _lib.ptr(alm2), k2,
+              alm2.stride(1) if k2 else 0, _lib.ptr(t.P), bp.lmax, _lib.ptr(coef), st)
+    _lib.call("gm_bat_syn", _lib.ptr(coef), k0, k2, bp.nside, bp.lmax, _lib.ptr(bp.alpha), _lib.ptr(t.smant),
+              _lib.ptr(t.sexp), _lib.ptr(bp.nact), bp.jsplit, _lib.ptr(spec0), spec0.shape[2] if k0 else 0,
+              spec0.stride(0) if k0 else 0, _lib.ptr(spec2), spec2.shape[2] if k2 else 0,
+              spec2.stride(0) if k2 else 0, st)
+
+
+def _fft_fwd(bp, maps0, maps2):
+    nside, lmax = bp.nside, bp.lmax
+    L, W, nring, npix = lmax + 1, 2 * lmax + 1, 4 * nside - 1, 12 * nside * nside
+    st = _lib.stream_ptr(bp.device)
+    spec0 = spec2 = None
+    if maps0 is not None:
+        spec0 = torch.empty((maps0.shape[0], nring, L), dtype=torch.complex128, device=bp.device)
+        _lib.call("gm_ring_fft_fwd_s0", _lib.ptr(maps0), maps0.shape[0], npix, nside, lmax, _lib.ptr(spec0), L,
+                  spec0.stride(0), st)
+    if maps2 is not None:
+        spec2 = torch.empty((maps2.shape[0], nring, W), dtype=torch.complex128, device=bp.device)
+        for f in range(maps2.shape[0]):
+            _lib.call("gm_ring_fft_fwd_s", _lib.ptr(maps2[f, 0]), _lib.ptr(maps2[f, 1]), nside, lmax,
+                      _lib.ptr(spec2[f]), W, st)
+    return spec0, spec2
+
+
+def _fold(bp, syn0, syn2, g0, g2, out0, out2):
+    """out = fold(syn) - g for every spectrum (the Jacobi residual in ring-Fourier space)."""
+    nside, lmax = bp.nside, bp.lmax
+    st = _lib.stream_ptr(bp.device)
+    if syn0 is not None:
+        _lib.call("gm_ring_fold_residual", _lib.ptr(syn0), _lib.ptr(g0), _lib.ptr(out0), syn0.shape[0], lmax + 1,
+                  syn0.stride(0), nside, lmax, st)
+    if syn2 is not None:
+        for f in range(syn2.shape[0]):
+            _lib.call("gm_ring_fold_residual_c", _lib.ptr(syn2[f]), _lib.ptr(g2[f]), _lib.ptr(out2[f]),
+                      syn2.shape[2], nside, lmax, st)
+
+
+def _like(t):
+    return None if t is None else torch.empty_like(t)
+
+
+def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True):
+    """``maps0`` ``(K0, npix)`` real spin-0 maps and ``maps2`` ``(K2, 2, npix)`` (Q, U) ->
+    ``(alm0 (K0, nelem), alm2 (K2, 2, nelem))`` with ``n_iter`` Jacobi steps; either input
+    may be None."""
+    dev = (maps0 if maps0 is not None else maps2).device
+    maps0 = None if maps0 is None else maps0.to(device=dev, dtype=torch.float64).contiguous()
+    maps2 = None if maps2 is None else maps2.to(device=dev, dtype=torch.float64).contiguous()
+    npix = (maps0 if maps0 is not None else maps2).shape[-1]
+    nside = sht._check_nside(npix)
+    if maps0 is not None and (maps0.ndim != 2 or maps0.shape[1] != npix):
+        raise ValueError("Input map has wrong shape")
+    if maps2 is not None and (maps2.ndim != 3 or maps2.shape[1] != 2 or maps2.shape[2] != npix):
+        raise ValueError("Input map has wrong shape")
+    k0 = 0 if maps0 is None else maps0.shape[0]
+    k2 = 0 if maps2 is None else maps2.shape[0]
+    _check(k0, k2)
+    lmax = int(lmax)
+    bp = plan(nside, lmax, dev, polar_skip)
+    L = lmax + 1
+    nelem = L * (L + 1) // 2
+    spec0, spec2 = _fft_fwd(bp, maps0, maps2)
+    alm0 = torch.empty((k0, nelem), dtype=torch.complex128, device=dev) if k0 else None
+    alm2 = torch.empty((k2, 2, nelem), dtype=torch.complex128, device=dev) if k2 else None
+    analysis(bp, spec0, spec2, alm0, alm2, 0)
+    if n_iter <= 0:
+        return alm0, alm2
+    syn0, syn2 = _like(spec0), _like(spec2)
+    ra0, ra2 = _like(spec0), _like(spec2)
+    synthesis(bp, alm0, alm2, syn0, syn2)
+    _fold(bp, syn0, syn2, spec0, spec2, ra0, ra2)
+    rb0, rb2 = spec0, spec2  # the input spectra are not read again
+    d0, d2 = _like(alm0), _like(alm2)
+    sign = 1
+    for it in range(int(n_iter)):
+        analysis(bp, ra0, ra2, d0, d2, 0)
+        for a, d in ((alm0, d0), (alm2, d2)):
+            if a is not None:
+                if sign > 0:
+                    a.sub_(d)
+                else:
+                    a.add_(d)
+        if it < n_iter - 1:
+            synthesis(bp, d0, d2, syn0, syn2)
+            _fold(bp, syn0, syn2, ra0, ra2, rb0, rb2)
+            ra0, rb0, ra2, rb2 = rb0, ra0, rb2, ra2
+            sign = -sign
+    return alm0, alm2
+
+
+def alm2map(alm0, alm2, nside, lmax, polar_skip=True):
+    """Inverse of ``map2alm`` (no refinement): ``(maps0 (K0, npix), maps2 (K2, 2, npix))``."""
+    dev = (alm0 if alm0 is not None else alm2).device
+    k0 = 0 if alm0 is None else alm0.shape[0]
+    k2 = 0 if alm2 is None else alm2.shape[0]
+    _check(k0, k2)
+    lmax = int(lmax)
+    bp = plan(nside, lmax, dev, polar_skip)
+    L, W, nring, npix = lmax + 1, 2 * lmax + 1, 4 * nside - 1, 12 * nside * nside
+    spec0 = torch.empty((k0, nring, L), dtype=torch.complex128, device=dev) if k0 else None
+    spec2 = torch.empty((k2, nring, W), dtype=torch.complex128, device=dev) if k2 else None
+    synthesis(bp, None if alm0 is None else alm0.contiguous(), None if alm2 is None else alm2.contiguous(),
+              spec0, spec2)
+    st = _lib.stream_ptr(dev)
+    maps0 = maps2 = None
+    if k0:
+        maps0 = torch.empty((k0, npix), dtype=torch.float64, device=dev)
+        _lib.call("gm_ring_fft_inv_s0", _lib.ptr(spec0), k0, L, spec0.stride(0), nside, lmax, _lib.ptr(maps0), npix, st)
+    if k2:
+        maps2 = torch.empty((k2, 2, npix), dtype=torch.float64, device=dev)
+        for f in range(k2):
+            _lib.call("gm_ring_fft_inv_s", _lib.ptr(spec2[f]), W, nside, lmax, _lib.ptr(maps2[f, 0]),
+                      _lib.ptr(maps2[f, 1]), st)
+    return maps0, maps2

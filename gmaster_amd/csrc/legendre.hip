@@ -20,24 +20,11 @@
 #include <type_traits>
 
 #include "common.h"
+#include "recurrence.h"
 
 namespace gm {
 
 namespace {
-
-// Extended-exponent constants per arithmetic type: a lane whose seed lies below 2^thr
-// starts with k < 0 and mu = lambda 2^(unit |k|) < 2^inactive; it is rescaled by 2^-unit
-// whenever |mu| passes 2^inactive.  FP32 keeps inactive lanes below 2^-36, far under its
-// 2^-24 resolution, and active seeds above 2^-100, inside its normal range.
-template <typename F> struct ExtExp;
-template <> struct ExtExp<double> {
-    static constexpr double inactive_max = 0x1p-300, rescale = 0x1p-600;
-    static constexpr int thr = -900, unit = 600;
-};
-template <> struct ExtExp<float> {
-    static constexpr float inactive_max = 0x1p-36f, rescale = 0x1p-64f;
-    static constexpr int thr = -100, unit = 64;
-};
 
 template <typename F> struct cpx { F x, y; };
 
@@ -45,27 +32,6 @@ template <typename F>
 __device__ inline cpx<F> load_c(const double2* p) {
     const double2 v = *p;
     return cpx<F>{(F)v.x, (F)v.y};
-}
-
-// lambda_M for order m at a ring as (mu, k): mant * 2^e from the host constant and
-// the powers sin^(m+s)(theta/2) cos^|m-s|(theta/2), evaluated in FP64.
-template <typename F>
-__device__ inline void seed_lane(double smant, int sexp, int a, int b, double shalf, double chalf, F& mu, int& k) {
-    double m1, m2;
-    int e1, e2;
-    pow_ext(shalf, a, m1, e1);
-    pow_ext(chalf, b, m2, e2);
-    const double v = smant * m1 * m2;
-    const int E = sexp + e1 + e2;
-    constexpr int thr = ExtExp<F>::thr, unit = ExtExp<F>::unit;
-    if (E >= thr) {
-        k = 0;
-        mu = (F)ldexp(v, E);
-    } else {
-        const int kk = (thr - E + unit - 1) / unit;
-        k = -kk;
-        mu = (F)ldexp(v, E + unit * kk);
-    }
 }
 
 // ------------------------------------------------------------------ block reduction
@@ -91,22 +57,6 @@ __device__ inline void block_reduce(const F (&acc)[NV], F* lds, int buf, double*
 #pragma unroll
     for (int msk = 1; msk < TPO; msk <<= 1) s += __shfl_xor(s, msk);
     if (sl == 0) out[o] = (double)s;
-}
-
-template <int R, typename F>
-__device__ inline void rescale(F (&mu0)[R], F (&mu1)[R], int (&kx)[R], bool& anyinact) {
-    if (!anyinact) return;
-    bool still = false;
-#pragma unroll
-    for (int t = 0; t < R; ++t) {
-        if (kx[t] < 0 && fabs(mu0[t]) > ExtExp<F>::inactive_max) {
-            mu0[t] *= ExtExp<F>::rescale;
-            mu1[t] *= ExtExp<F>::rescale;
-            kx[t] += 1;
-        }
-        still = still || (kx[t] < 0);
-    }
-    anyinact = __any(still);
 }
 
 // ------------------------------------------------------------------ spin-0 analysis

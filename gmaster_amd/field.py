@@ -12,7 +12,7 @@ purified ones.  Anisotropic masks are outside this engine and raise ``NotImpleme
 
 import torch
 
-from . import _lib, sht
+from . import _lib, batched, sht
 from ._array import device_tensor, wrap
 from .deproject import deproject
 from .purify import purify
@@ -57,12 +57,98 @@ def _remember(key, alm):
     _MASK_ALMS[key] = alm
 
 
+# Fields whose transforms wait for the batched pass (fast form): each entry is the field,
+# its spin-0 channels ((K, npix) masked map and optionally its mask, copied) or its (Q, U)
+# maps, and the mask-alms cache key when the mask rides along.  The first access to any
+# pending field's alms runs every pending transform, grouped by geometry and n_iter into
+# the supported (spin-0 maps, spin-2 fields) batches.
+_PENDING = []
+
+
+def _batchable(nside, lmax):
+    return nmt_params.form == "fast" and nmt_params.batch_fields and lmax >= 2 and nside <= 4096
+
+
+def flush_pending():
+    """Run every deferred field transform (called on the first access to pending alms)."""
+    if not _PENDING:
+        return
+    todo = list(_PENDING)
+    _PENDING.clear()
+    groups = {}
+    for item in todo:
+        f = item["field"]
+        key = (f.minfo.nside, f.ainfo.lmax, f.n_iter, str(f.mask.device))
+        groups.setdefault(key, []).append(item)
+    for (nside, lmax, n_iter, _), items in groups.items():
+        # channel lists: spin-0 (field, slot) with slot "alm" or "mask"; spin-2 fields
+        c0, c2 = [], []
+        for it in items:
+            for k, what in enumerate(it["what0"]):
+                c0.append((it, k, what))
+            if it["maps2"] is not None:
+                c2.append(it)
+        while c0 or c2:
+            if c2:
+                n0 = min(2, len(c0))
+                n2 = 1 if n0 or len(c2) == 1 else 2
+            else:
+                n0 = next(k for k in (8, 4, 2, 1) if k <= len(c0))
+                n2 = 0
+            take0, c0 = c0[:n0], c0[n0:]
+            take2, c2 = c2[:n2], c2[n2:]
+            if n0 == 1 and n2 == 0:  # a lone spin-0 map: the per-spin mixed transform
+                it, k, what = take0[0]
+                a = sht.map2alm_s0(it["maps0"][k:k + 1], lmax, n_iter=n_iter, precision="mixed")[0]
+                _deliver(it, what, a)
+                continue
+            m0 = torch.stack([it["maps0"][k] for it, k, _ in take0]) if take0 else None
+            m2 = torch.stack([it["maps2"] for it in take2]) if take2 else None
+            a0, a2 = batched.map2alm(m0, m2, lmax, n_iter=n_iter)
+            for i, (it, k, what) in enumerate(take0):
+                _deliver(it, what, a0[i].clone() if n0 > 1 else a0[i])
+            for i, it in enumerate(take2):
+                it["field"]._alm = a2[i].clone() if n2 > 1 else a2[i]
+    for it in todo:
+        it["field"]._pending = False
+
+
+def _deliver(item, what, alm):
+    f = item["field"]
+    if what == "alm":
+        f._alm = alm[None]
+    else:
+        f._alm_mask = alm
+        _remember(item["mask_key"], alm)
+
+
 class NmtField:
+    @property
+    def alm(self):
+        if self._pending:
+            flush_pending()
+        return self._alm
+
+    @alm.setter
+    def alm(self, value):
+        self._alm = value
+
+    @property
+    def alm_mask(self):
+        if self._pending:
+            flush_pending()
+        return self._alm_mask
+
+    @alm_mask.setter
+    def alm_mask(self, value):
+        self._alm_mask = value
+
     def __init__(self, mask, maps, *, spin=None, templates=None, beam=None, purify_e=False, purify_b=False,
                  n_iter=None, n_iter_mask=None, tol_pinv=None, wcs=None, lmax=None, lmax_mask=None,
                  masked_on_input=False, lite=False, mask_22=None, mask_12=None):
         if mask_22 is not None or mask_12 is not None:
             raise NotImplementedError("anisotropic masks are not supported by this engine")
+        self._pending = False
         self.lite = bool(lite)
         self.n_iter = nmt_params.n_iter_default if n_iter is None else int(n_iter)
         self.n_iter_mask = nmt_params.n_iter_mask_default if n_iter_mask is None else int(n_iter_mask)
@@ -166,6 +252,21 @@ class NmtField:
             amask = self.get_mask_alms().as_subclass(torch.Tensor)
             self.alm, masked = purify(self.mask, amask, unmasked, lmax, (self.pure_e, self.pure_b), self.n_iter,
                                       precision=_precision())
+        elif temps is None and _batchable(self.minfo.nside, lmax):
+            # deferred: the transform joins the next batched pass (with the mask when it
+            # would be paired, as below)
+            pair = (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
+                    and self.n_iter == self.n_iter_mask and self._alm_mask is None
+                    and not any(it["mask_key"] == key and "mask" in it["what0"] for it in _PENDING))
+            own = masked.clone() if masked is maps_t else masked  # never the caller's tensor
+            item = {"field": self, "mask_key": key, "maps0": None, "maps2": None, "what0": []}
+            if self.spin == 0:
+                item["maps0"] = torch.cat([masked, self.mask[None, :]]) if pair else own
+                item["what0"] = ["alm", "mask"] if pair else ["alm"]
+            else:
+                item["maps2"] = own
+            _PENDING.append(item)
+            self._pending = True
         elif (self.spin == 0 and not self.lite and self.ainfo == self.ainfo_mask
                 and self.n_iter == self.n_iter_mask and self.alm_mask is None):
             pair = sht.map2alm_s0(torch.cat([masked, self.mask[None, :]]), lmax, n_iter=self.n_iter,

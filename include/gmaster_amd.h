@@ -158,6 +158,44 @@ int64_t gm_cl_work_size(int64_t lmax, int64_t npairs);
 int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t n2, int64_t alm_stride,
                     int64_t lmax, double* cl, double* work, void* stream);
 
+/* ---- batched latitudinal pass: K0 real spin-0 maps and K2 spin-2 fields sharing one
+ * spin-0 recurrence lambda_lm per (m, ring pair), contraction on the matrix unit.
+ * Supported (K0, K2): (2, 1), (1, 1), (0, 1), (0, 2), (2, 0), (4, 0), (8, 0).  Tables
+ * (alpha, P, seeds) are the spin-0 ones of lmax + 1 (gm_lat_tables(lmax + 1, 0, ...)): the
+ * spin-2 rows need lambda up to degree lmax + 1.  nactive[m] counts ring pairs (equator
+ * first); pairs j >= jsplit (near the poles) contract in FP64, the others in FP32.
+ * The kernels read whole 32-degree blocks: alpha must be followed by 64 zero doubles and
+ * the synthesis coefficients by 64 spare degrees (64 x gm_bat_stride(.., 1) doubles).
+ * Replaces, for this batched form, the march analysis/synthesis FFI handlers
+ * gm_march_ana_s0_pair / gm_march_ana_s2 / gm_march_syn_s0_pair / gm_march_syn_s2
+ * (gmaster/_native/cuda/march_v2.cu:496-594), which run one recurrence per spin. */
+
+/* Ring blocks of the FP32 (polar = 0) or FP64 (polar = 1) analysis launch; -1 if invalid. */
+int64_t gm_bat_blocks(int64_t nside, int64_t jsplit, int64_t polar);
+/* Channel stride of the analysis partial sums (synthesis = 0) or synthesis coefficients (1). */
+int64_t gm_bat_stride(int64_t k0, int64_t k2, int64_t synthesis);
+/* Analysis partial sums: spec0 (k0, nring, ld0 >= lmax+1) phased spectra of real maps,
+ * spec2 (k2, nring, ld2 >= 2 lmax + 1) phased spectra F+ of Q + iU (column lmax + m);
+ * part32: gm_bat_blocks(.., 0) x gm_table_size(lmax + 1) x stride floats; part64, the
+ * polar blocks, gm_bat_blocks(.., 1) x T x stride doubles with T the table entries of the
+ * orders m < mpol (every order with nactive[m] > jsplit). */
+int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, const double* spec2, int64_t k2,
+               int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax, const double* alpha, const double* seed_mant,
+               const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit, int64_t mpol, float* part32,
+               double* part64, void* stream);
+/* Sums the partials, applies P_l and forms spin-0 alms (alm0 + k astride0) and E, B
+ * (alm2 + (2 f + c) astride2, c = 0 for E, 1 for B); mode 0 writes, 1 subtracts, 2 adds. */
+int gm_bat_finish(const float* part32, const double* part64, int64_t k0, int64_t k2, int64_t nside, int64_t lmax,
+                  const int32_t* nactive, int64_t jsplit, int64_t mpol, const double* P, double* alm0,
+                  int64_t astride0, double* alm2, int64_t astride2, int mode, void* stream);
+/* Synthesis coefficients (gm_table_size(lmax + 1) x gm_bat_stride(.., 1) doubles). */
+int gm_bat_prep(const double* alm0, int64_t k0, int64_t astride0, const double* alm2, int64_t k2, int64_t astride2,
+                const double* P, int64_t lmax, double* coef, void* stream);
+/* Synthesis of every ring pair into spec0 / spec2 (layouts as for gm_bat_ana). */
+int gm_bat_syn(const double* coef, int64_t k0, int64_t k2, int64_t nside, int64_t lmax, const double* alpha,
+               const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit,
+               double* spec0, int64_t ld0, int64_t smap0, double* spec2, int64_t ld2, int64_t smap2, void* stream);
+
 /* gm_coupled_cell without a caller workspace and without any allocation: the partial sums
  * use a 64 MiB device array of the library (gm_coupled_cell_noalloc_max_work() doubles per
  * GPU, held until the process ends), so lmax <= 8191 with 2 x 2 components (lmax <= 16383
