@@ -32,8 +32,8 @@ SUPPORTED = ((2, 1), (1, 1), (0, 1), (0, 2), (2, 0), (4, 0), (8, 0))
 
 # zero entries after the alpha table and spare degrees after the synthesis coefficients
 # (the kernels read whole 32-degree blocks past an order's last degree)
-ALPHA_TAIL = 64
-COEF_TAIL = 64
+ALPHA_TAIL = 128
+COEF_TAIL = 96
 
 
 @dataclass
@@ -41,7 +41,7 @@ class BatPlan:
     nside: int
     lmax: int
     device: torch.device
-    tab: sht.Plan          # spin-0 tables of lmax + 1 (alpha, P valid up to degree lmax + 1)
+    tab: sht.Plan          # spin-0 tables of lmax + 2 (alpha, P valid up to degree lmax + 2)
     alpha: torch.Tensor    # tab.alpha followed by ALPHA_TAIL zeros (whole-block reads stay in bounds)
     nact: torch.Tensor     # ring pairs visited by order m (prefix from the equator)
     jsplit: int            # first ring pair contracted in FP64
@@ -83,11 +83,11 @@ def plan(nside, lmax, device=None, polar_skip=True):
     if bp is not None:
         return bp
     nside, lmax = int(nside), int(lmax)
-    tab = sht.plan(nside, lmax + 1, 0, device, polar_skip)
+    tab = sht.plan(nside, lmax + 2, 0, device, polar_skip)
     jsplit = polar_split(nside)
     nact = _nactive_pairs(nside, lmax, polar_skip)
     mpol = int(np.count_nonzero(nact > jsplit))
-    Lt = lmax + 2
+    Lt = lmax + 3  # table degrees
     alpha = torch.zeros(tab.alpha.numel() + ALPHA_TAIL, dtype=torch.float64, device=device)
     alpha[: tab.alpha.numel()].copy_(tab.alpha)
     bp = BatPlan(
@@ -96,7 +96,7 @@ def plan(nside, lmax, device=None, polar_skip=True):
         jsplit=jsplit, mpol=mpol, tsize64=mpol * (Lt + 8) - mpol * (mpol - 1) // 2,
         nb32=int(_lib.query("gm_bat_blocks", nside, jsplit, 0)),
         nb64=int(_lib.query("gm_bat_blocks", nside, jsplit, 1)),
-        tsize=int(_lib.query("gm_table_size", lmax + 1)),
+        tsize=int(_lib.query("gm_table_size", lmax + 2)),
     )
     _PLANS[key] = bp
     return bp

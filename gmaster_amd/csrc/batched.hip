@@ -10,9 +10,11 @@
 // An analysis therefore needs, per order m and degree j = m..L, only the sums
 //     S0_j(G) = sum_r lambda_j(theta_r) G_r     and     S1_j(G) = sum_r lambda_j(theta_r) G_r / s_r^2
 // of the ring spectra G of Q and U (real maps: G_Q, G_U from the +-m columns of Q + iU),
-// and a synthesis only sums of lambda_j against per-degree coefficients, split into a
-// 1/s^2 part and a plain part.  T, the mask and Q/U of one realisation are then 12 real
-// channels of one recurrence instead of three recurrences of 4 channels.
+// and a synthesis only sums of lambda_j against per-degree coefficients: its plain part
+// folds into the 1/s^2 part through s^2 lambda_j = lambda_j - x^2 lambda_j (a combination of
+// lambda_{j-2}, lambda_j, lambda_{j+2}), so Q(theta) = sum_j gamma_j lambda_j / s^2 with one
+// coefficient per degree.  T, the mask and Q/U of one realisation are then 12 real analysis
+// channels (8 synthesis channels) of one recurrence instead of three recurrences of 4.
 //
 // Analysis (k_bat_ana): lanes are folded ring pairs (north and south rings of one |z|,
 // combined as E = N + S for even l + m and O = N - S for odd).  Every 32 degrees each lane
@@ -30,9 +32,10 @@
 // differently from the f32 instruction's); elsewhere 1/s^2 <= 16 and FP32 keeps ~1e-7.
 //
 // Synthesis (k_bat_syn): as the spin-0 mixed synthesis, one v_mfma_f32_4x4x1f32 per
-// (degree, ring set, 4 channels), lanes = ring pairs, E/O accumulators; the epilogue forms
-// Q = A/s^2 + B and U likewise and writes F+(+-m) = Q + iU, conj(Q) + i conj(U).  Polar
-// ring pairs use an FP64 vector-ALU variant (k_bat_syn64).
+// (degree, ring set, 4 channels), lanes = ring pairs, E/O accumulators; the epilogue
+// divides the spin-2 sums by s^2 and writes F+(+-m) = Q + iU, conj(Q) + i conj(U).  Polar
+// ring pairs use an FP64 vector-ALU variant (k_bat_syn64).  Tables run to degree lmax + 2
+// (the folded plain part reaches one degree past lmax + 1).
 #include "common.h"
 #include "recurrence.h"
 
@@ -75,23 +78,30 @@ template <> struct Mfma16<double> {
 #define GM_BAT_SYN_WAVES 4  // minimum waves per SIMD the FP32 synthesis is compiled for
 #endif
 
-constexpr int kTile = 32;   // degrees per analysis tile (16 of each parity)
-constexpr int kAlphaTail = 64;  // zero doubles the caller appends to the alpha table
-constexpr int kCoefTail = 2 * kTile;  // zero degrees (x the channel stride) appended to the coefficients
+constexpr int kTile = 32;       // degrees per analysis tile (16 of each parity)
+constexpr int kSuper = 2;       // analysis tiles per cross-wave reduction
+constexpr int kTabExtra = 2;    // table degrees beyond lmax + 1 (tables of lmax + 2)
+constexpr int kAlphaTail = 128;  // zero doubles the caller appends to the alpha table
+constexpr int kCoefTail = 3 * kTile;  // spare degrees (x the channel stride) after the coefficients
 constexpr int kRow = 68;    // transpose row: 64 rings + 4 padding
 
 // Channel counts.  Analysis: spin-0 map k at slots 2k, 2k+1 (re, im); spin-2 field f at
 // 2 K0 + 8 f + (S0Q re, im, S0U re, im, S1Q re, im, S1U re, im).  Synthesis: spin-0 as
-// in the analysis; spin-2 field f at syn_base2 + 8 f + (AQ re, im, AU re, im, BQ re, im,
-// BU re, im), aligned to 4 so that the 1/s^2 part and the plain part of one component sit
-// in the same lane of a 4x4x1 block.
+// in the analysis; spin-2 field f at syn_base2 + 4 f + (Q re, im, U re, im), aligned to 4
+// so that the four components of one field sit in the four lanes of a 4x4x1 block.
 template <int K0, int K2> struct Chan {
     static constexpr int nc = 2 * K0 + 8 * K2;          // analysis channels
     static constexpr int ana_stride = (nc + 3) / 4 * 4;  // partial-sum stride
     static constexpr int syn_base2 = (2 * K0 + 3) / 4 * 4;
-    static constexpr int syn_stride = syn_base2 + 8 * K2;  // coefficient stride (multiple of 4)
+    static constexpr int syn_stride = syn_base2 + 4 * K2;  // coefficient stride (multiple of 4)
     static_assert(nc <= 16, "at most 16 analysis channels");
 };
+
+__device__ inline double eps_lm(int j, int m) {
+    if (j < m || j <= 0) return 0.0;
+    const double dj = j, dm = m;
+    return sqrt((dj * dj - dm * dm) / (4.0 * dj * dj - 1.0));
+}
 
 // The five per-(l, m) coefficients of W_l and X_l (zero below max(m, 2)).
 __device__ inline void wx_coefs(int l, int m, double& aW, double& bW, double& cW, double& dX, double& eX) {
@@ -172,7 +182,7 @@ template <> struct Vec4<double> { typedef d4 type; };
 // Workgroup (blockIdx.x = ring block, blockIdx.y = m) over ring pairs
 // [jbeg + blockIdx.x * CH, +CH) ∩ [jbeg, min(nact[m], jend)); degrees m..L (L = lmax + 1:
 // S1_{lmax+1} enters X_lmax).  Writes partial[blockIdx.x][table entry of (m, l)][slot] for
-// l <= L, tables of L + 1 degrees (alpha valid for l <= L and zero up to L + GM_TABLE_PAD).
+// l <= L, tables of L + kTabExtra degrees (alpha valid for l <= L + 1).
 template <int K0, int K2, int NW, int R, typename F>
 __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(const double2* __restrict__ spec0, int64_t ld0, int64_t smap0,
                                                      const double2* __restrict__ spec2, int64_t ld2, int64_t smap2,
@@ -187,7 +197,7 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
     constexpr int CH = T * R;
     constexpr int SLOT = C::ana_stride;
     __shared__ __align__(32) F tb[NW][2 * 16 * kRow];
-    const int Lt = L + 1;
+    const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
     const int jhi = min(nact[m], jend);
@@ -230,80 +240,85 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
     }
 
     // alpha_l at al[l]; the table array carries kAlphaTail zero entries past its end, so a
-    // tile's reads up to degree l0 + 32 stay in bounds for every order (past L they are the
-    // next order's entries or zeros, and only feed degrees above L, which are never stored)
+    // super-tile's reads up to degree l0 + 64 stay in bounds for every order (past the
+    // table they are the next order's entries or zeros, and only feed degrees above L,
+    // which are never stored)
     const double* al = alpha + table_base(m, Lt) - m;
     F* part = partial + ((int64_t)blockIdx.x * tsize + table_base(m, Lt) - m) * SLOT;
     F* mytb = tb[w];
     const int col = ln;  // transposed column of this lane's ring (4 (col % 16) + col / 16)
-    for (int l0 = m; l0 <= L; l0 += kTile) {
-        double a[kTile];
+    for (int ls = m; ls <= L; ls += kSuper * kTile) {
+        acc_t acc[kSuper][2];
 #pragma unroll
-        for (int i = 0; i < kTile; ++i) a[i] = al[l0 + i + 1];
-        acc_t acc[2];
+        for (int h = 0; h < kSuper; ++h) {
+            const int l0 = ls + h * kTile;
+            double a[kTile];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[0][v] = acc[1][v] = (F)0;
+            for (int i = 0; i < kTile; ++i) a[i] = al[l0 + i + 1];
 #pragma unroll
-        for (int t = 0; t < R; ++t) {
-            if (!live[t]) continue;
-            // recurrence: mu of degrees l0 .. l0 + 31 into the tile (row = parity, degree / 2)
+            for (int v = 0; v < 4; ++v) acc[h][0][v] = acc[h][1][v] = (F)0;
 #pragma unroll
-            for (int i4 = 0; i4 < kTile / 4; ++i4) {
+            for (int t = 0; t < R; ++t) {
+                if (!live[t]) continue;
+                // recurrence: mu of degrees l0 .. l0 + 31 into the tile (row = parity, degree / 2)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int i = 4 * i4 + u;
-                    const double cur = (i & 1) ? mu1[t] : mu0[t];
-                    mytb[((i & 1) * 16 + (i >> 1)) * kRow + col] = (F)cur;
-                    if (i & 1) mu0[t] = fma(x[t] * a[i], mu1[t], -mu0[t]);
-                    else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
-                }
-                if (inact[t]) {
-                    if (kx[t] < 0 && fabs(mu0[t]) > ExtExp<double>::inactive_max) {
-                        mu0[t] *= ExtExp<double>::rescale;
-                        mu1[t] *= ExtExp<double>::rescale;
-                        kx[t] += 1;
+                for (int i4 = 0; i4 < kTile / 4; ++i4) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int i = 4 * i4 + u;
+                        const double cur = (i & 1) ? mu1[t] : mu0[t];
+                        mytb[((i & 1) * 16 + (i >> 1)) * kRow + col] = (F)cur;
+                        if (i & 1) mu0[t] = fma(x[t] * a[i], mu1[t], -mu0[t]);
+                        else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
                     }
-                    inact[t] = __any(kx[t] < 0);
+                    if (inact[t]) {
+                        if (kx[t] < 0 && fabs(mu0[t]) > ExtExp<double>::inactive_max) {
+                            mu0[t] *= ExtExp<double>::rescale;
+                            mu1[t] *= ExtExp<double>::rescale;
+                            kx[t] += 1;
+                        }
+                        inact[t] = __any(kx[t] < 0);
+                    }
                 }
-            }
-            wave_sync();
-            // lane (kq, n = ich) reads mu(degree 2n + p, ring 4q + kq) for q = 4g .. 4g + 3
+                wave_sync();
+                // lane (kq, n = ich) reads mu(degree 2n + p, ring 4q + kq) for q = 4g .. 4g + 3
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
+                for (int g = 0; g < 4; ++g) {
 #pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    const v4 b = *reinterpret_cast<const v4*>(mytb + (p * 16 + ich) * kRow + kq * 16 + 4 * g);
+                    for (int p = 0; p < 2; ++p) {
+                        const v4 b = *reinterpret_cast<const v4*>(mytb + (p * 16 + ich) * kRow + kq * 16 + 4 * g);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) acc[p] = Mfma16<F>::op(av[t][p][4 * g + u], b[u], acc[p]);
+                        for (int u = 0; u < 4; ++u) acc[h][p] = Mfma16<F>::op(av[t][p][4 * g + u], b[u], acc[h][p]);
+                    }
                 }
+                wave_sync();
             }
-            wave_sync();
         }
-        // acc[p][v] at lane (kq, n): channel Mfma16<F>::row(kq, v), degree l0 + 2 n + p.  Each
-        // wave parks its tile in its own (now free) transpose buffer, then the waves sum.
+        // acc[h][p][v] at lane (kq, n): channel Mfma16<F>::row(kq, v), degree ls + 32 h + 2 n + p.
+        // Each wave parks its super-tile in its own (now free) transpose buffer in output
+        // order [degree][slot], then the waves sum slot by slot (consecutive threads read
+        // consecutive words).
+        constexpr int NOUT = kSuper * kTile * SLOT;
+        static_assert(NOUT <= 2 * 16 * kRow, "super-tile must fit the transpose buffer");
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            v4 o;
+        for (int h = 0; h < kSuper; ++h)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) o[v] = acc[p][v];
-            *reinterpret_cast<v4*>(mytb + (p * 64 + ln) * 4) = o;
-        }
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = Mfma16<F>::row(kq, v);
+                    if (c < SLOT) mytb[(h * kTile + 2 * ich + p) * SLOT + c] = acc[h][p][v];
+                }
         __syncthreads();
-        for (int u = threadIdx.x; u < kTile * SLOT; u += T) {
-            const int d = u / SLOT, c = u % SLOT;
-            const int l = l0 + d;
+        for (int u = threadIdx.x; u < NOUT; u += T) {
+            const int l = ls + u / SLOT;
             if (l > L) continue;
-            const int p = d & 1, n = d >> 1;
-            // lane group and register holding channel c
-            const int grp = Mfma16<F>::row(1, 0) == 4 ? c >> 2 : c & 3;
-            const int reg = Mfma16<F>::row(1, 0) == 4 ? c & 3 : c >> 2;
-            const int idx = (p * 64 + 16 * grp + n) * 4 + reg;
-            F s = 0;
+            F sum = 0;
 #pragma unroll
-            for (int ww = 0; ww < NW; ++ww) s += tb[ww][idx];
-            part[(int64_t)l * SLOT + c] = s;
+            for (int ww = 0; ww < NW; ++ww) sum += tb[ww][u];
+            part[(int64_t)ls * SLOT + u] = sum;
         }
-        __syncthreads();  // the next tile overwrites the buffers
+        __syncthreads();  // the next super-tile overwrites the buffers
     }
 }
 
@@ -325,7 +340,7 @@ __global__ __launch_bounds__(256) void k_bat_finish(const float* __restrict__ p3
     constexpr int SLOT = C::ana_stride;
     constexpr int NH = 256 + 4;  // degrees lb - 2 .. lb + 257
     __shared__ double S[NH][SLOT];
-    const int Lt = L + 1;
+    const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int lb = m + blockIdx.x * 256;
     if (lb > L - 1) return;
@@ -392,24 +407,27 @@ __global__ __launch_bounds__(256) void k_bat_finish(const float* __restrict__ p3
 }
 
 // ------------------------------------------------------------------ synthesis
-// Coefficients per (m, degree j = m .. L) and channel (synthesis layout), times P_j:
-//   spin 0: a_jm;  spin 2: AQ_j = -[aW_j E_j + cW_{j+2} E_{j+2} + i (dX_{j-1} B_{j-1} + eX_{j+1} B_{j+1})],
-//   AU_j = -[aW_j B_j + cW_{j+2} B_{j+2} - i (dX_{j-1} E_{j-1} + eX_{j+1} E_{j+1})],
-//   BQ_j = -bW_j E_j, BU_j = -bW_j B_j, so that Q(theta) = sum_j (AQ_j / s^2 + BQ_j) lambda_j.
+// Coefficients per (m, degree j = m .. L + 1) and channel (synthesis layout), times P_j:
+//   spin 0: a_jm;  spin 2: Q(theta) = (1/s^2) sum_j GQ_j lambda_j with
+//   GQ_j = AQ_j + BQ_j (1 - eps_{j+1}^2 - eps_j^2) - BQ_{j-2} eps_{j-1} eps_j - BQ_{j+2} eps_{j+1} eps_{j+2},
+//   AQ_j = -[aW_j E_j + cW_{j+2} E_{j+2} + i (dX_{j-1} B_{j-1} + eX_{j+1} B_{j+1})],  BQ_j = -bW_j E_j
+//   (the plain part sum_j BQ_j lambda_j rewritten through s^2 lambda_j = lambda_j - x^2 lambda_j),
+//   and GU_j likewise from AU_j = -[aW_j B_j + cW_{j+2} B_{j+2} - i (dX_{j-1} E_{j-1} + eX_{j+1} E_{j+1})],
+//   BU_j = -bW_j B_j.
 template <int K0, int K2>
 __global__ __launch_bounds__(256) void k_bat_prep(const double2* __restrict__ alm0, int64_t as0,
                                                   const double2* __restrict__ alm2, int64_t as2,
                                                   const double* __restrict__ P, int L, double* __restrict__ coef) {
     typedef Chan<K0, K2> C;
     constexpr int NS = C::syn_stride;
-    const int Lt = L + 1;
+    const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int lmax = L - 1;
     const int64_t pb = table_base(m, Lt) - m;
     const int l = m + blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= Lt + GM_TABLE_PAD) return;
     double* out = coef + (pb + l) * NS;
-    const double pl = l <= L ? P[pb + l] : 0.0;
+    const double pl = l <= L + 1 ? P[pb + l] : 0.0;
 #pragma unroll
     for (int k = 0; k < K0; ++k) {
         double2 a = make_double2(0.0, 0.0);
@@ -425,44 +443,47 @@ __global__ __launch_bounds__(256) void k_bat_prep(const double2* __restrict__ al
             if (j < lo || j > lmax) return make_double2(0.0, 0.0);
             return alm2[(2 * f + comp) * as2 + alm_index(j, m, lmax)];
         };
-        double aQx = 0, aQy = 0, aUx = 0, aUy = 0, bQx = 0, bQy = 0, bUx = 0, bUy = 0;
-        if (l <= L) {
-            double aW, bW, cW, dX, eX, t1, t2, t3, t4;
-            wx_coefs(l, m, aW, bW, cW, t1, t2);
-            double cW2;
-            wx_coefs(l + 2, m, t1, t2, cW2, t3, t4);
-            double dXm;
-            wx_coefs(l - 1, m, t1, t2, t3, dXm, t4);
-            double eXp;
-            wx_coefs(l + 1, m, t1, t2, t3, t4, eXp);
-            (void)dX;
-            (void)eX;
-            const double2 E0 = eb(0, l), B0 = eb(1, l), E2 = eb(0, l + 2), B2 = eb(1, l + 2);
-            const double2 Em = eb(0, l - 1), Bm = eb(1, l - 1), Ep = eb(0, l + 1), Bp = eb(1, l + 1);
-            // w = aW X_l + cW_{l+2} X_{l+2};  x = dX_{l-1} Y_{l-1} + eX_{l+1} Y_{l+1}
+        // A part at degree j (1/s^2 coefficient before folding)
+        auto apart = [&](int j, double2& aq, double2& au) {
+            double aW, bW, cW, dX, eX, t1, t2, t3, t4, cW2, dXm, eXp;
+            wx_coefs(j, m, aW, bW, cW, dX, eX);
+            wx_coefs(j + 2, m, t1, t2, cW2, t3, t4);
+            wx_coefs(j - 1, m, t1, t2, t3, dXm, t4);
+            wx_coefs(j + 1, m, t1, t2, t3, t4, eXp);
+            const double2 E0 = eb(0, j), B0 = eb(1, j), E2 = eb(0, j + 2), B2 = eb(1, j + 2);
+            const double2 Em = eb(0, j - 1), Bm = eb(1, j - 1), Ep = eb(0, j + 1), Bp = eb(1, j + 1);
             const double wEx = aW * E0.x + cW2 * E2.x, wEy = aW * E0.y + cW2 * E2.y;
             const double wBx = aW * B0.x + cW2 * B2.x, wBy = aW * B0.y + cW2 * B2.y;
             const double xBx = dXm * Bm.x + eXp * Bp.x, xBy = dXm * Bm.y + eXp * Bp.y;
             const double xEx = dXm * Em.x + eXp * Ep.x, xEy = dXm * Em.y + eXp * Ep.y;
-            // AQ = -(wE + i xB), AU = -(wB - i xE)
-            aQx = -(wEx - xBy);
-            aQy = -(wEy + xBx);
-            aUx = -(wBx + xEy);
-            aUy = -(wBy - xEx);
-            bQx = -bW * E0.x;
-            bQy = -bW * E0.y;
-            bUx = -bW * B0.x;
-            bUy = -bW * B0.y;
+            aq = make_double2(-(wEx - xBy), -(wEy + xBx));  // -(wE + i xB)
+            au = make_double2(-(wBx + xEy), -(wBy - xEx));  // -(wB - i xE)
+        };
+        // plain part at degree j: BQ_j = -bW_j E_j, BU_j = -bW_j B_j
+        auto bpart = [&](int j, double2& bq, double2& bu) {
+            double aW, bW, cW, dX, eX;
+            wx_coefs(j, m, aW, bW, cW, dX, eX);
+            const double2 E0 = eb(0, j), B0 = eb(1, j);
+            bq = make_double2(-bW * E0.x, -bW * E0.y);
+            bu = make_double2(-bW * B0.x, -bW * B0.y);
+        };
+        double2 gq = make_double2(0.0, 0.0), gu = gq;
+        if (l <= L + 1) {
+            double2 aq, au, bq, bu, bqm, bum, bqp, bup;
+            apart(l, aq, au);
+            bpart(l, bq, bu);
+            bpart(l - 2, bqm, bum);
+            bpart(l + 2, bqp, bup);
+            const double e0 = eps_lm(l, m), e1 = eps_lm(l + 1, m), e2 = eps_lm(l + 2, m), em = eps_lm(l - 1, m);
+            const double w0 = 1.0 - e1 * e1 - e0 * e0, wm = -em * e0, wp = -e1 * e2;
+            gq = make_double2(aq.x + w0 * bq.x + wm * bqm.x + wp * bqp.x, aq.y + w0 * bq.y + wm * bqm.y + wp * bqp.y);
+            gu = make_double2(au.x + w0 * bu.x + wm * bum.x + wp * bup.x, au.y + w0 * bu.y + wm * bum.y + wp * bup.y);
         }
-        double* o = out + C::syn_base2 + 8 * f;
-        o[0] = aQx * pl;
-        o[1] = aQy * pl;
-        o[2] = aUx * pl;
-        o[3] = aUy * pl;
-        o[4] = bQx * pl;
-        o[5] = bQy * pl;
-        o[6] = bUx * pl;
-        o[7] = bUy * pl;
+        double* o = out + C::syn_base2 + 4 * f;
+        o[0] = gq.x * pl;
+        o[1] = gq.y * pl;
+        o[2] = gu.x * pl;
+        o[3] = gu.y * pl;
     }
 }
 
@@ -494,7 +515,7 @@ __global__ __launch_bounds__(64 * NW, GM_BAT_SYN_WAVES) void k_bat_syn(const dou
     static_assert(kTile * NS % 64 == 0, "a block of coefficients must split evenly over a wave");
     // per wave: the FP32 coefficients of kTile degrees, [channel][degree]
     __shared__ __align__(16) float cst[NW][NS * SR];
-    const int Lt = L + 1;
+    const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int jhi = min(nact[m], jend);
     const int j0 = jbeg + blockIdx.x * CH;
@@ -538,12 +559,12 @@ __global__ __launch_bounds__(64 * NW, GM_BAT_SYN_WAVES) void k_bat_syn(const dou
         double an[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) an[i] = al[m + i + 1];
-        for (int lb = m; lb <= L; lb += kTile) {
+        for (int lb = m; lb <= L + 1; lb += kTile) {
             wave_sync();  // the previous block's reads are done
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 const int e = v * 64 + ln, d = e / NS, ch = e % NS;
-                my[ch * SR + d] = lb + d <= L ? (float)pf[v] : 0.f;
+                my[ch * SR + d] = lb + d <= L + 1 ? (float)pf[v] : 0.f;
             }
             wave_sync();
             for (int i4 = 0; i4 < kTile / 4; ++i4) {
@@ -609,10 +630,10 @@ __global__ __launch_bounds__(64 * NW, GM_BAT_SYN_WAVES) void k_bat_syn(const dou
             }
 #pragma unroll
             for (int f = 0; f < K2; ++f) {
-                const int qa = C::syn_base2 / 4 + 2 * f, qb = qa + 1;
+                const int qa = C::syn_base2 / 4 + f;
                 // component q4 (Q re, Q im, U re, U im) of the north and south rings
-                const double n = (double)(acc[0][t][qa][v] + acc[1][t][qa][v]) * rs + (double)(acc[0][t][qb][v] + acc[1][t][qb][v]);
-                const double s = (double)(acc[0][t][qa][v] - acc[1][t][qa][v]) * rs + (double)(acc[0][t][qb][v] - acc[1][t][qb][v]);
+                const double n = ((double)acc[0][t][qa][v] + (double)acc[1][t][qa][v]) * rs;
+                const double s = ((double)acc[0][t][qa][v] - (double)acc[1][t][qa][v]) * rs;
                 // lane 0 of the quad needs U im (lane 3), lane 1 U re (lane 2)
                 const double n_x = __shfl_xor(n, 3), s_x = __shfl_xor(s, 3);
                 if (!ok) continue;
@@ -651,7 +672,7 @@ __global__ __launch_bounds__(64 * NW) void k_bat_syn64(const double* __restrict_
     typedef Chan<K0, K2> C;
     constexpr int NS = C::syn_stride;
     constexpr int T = 64 * NW;
-    const int Lt = L + 1;
+    const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int jhi = min(nact[m], jend);
     const int j0 = jbeg + blockIdx.x * T * R;
@@ -680,13 +701,13 @@ __global__ __launch_bounds__(64 * NW) void k_bat_syn64(const double* __restrict_
         anyinact = __any(anyinact);
         const double* al = alpha + table_base(m, Lt) - m;
         const double* cf = coef + (table_base(m, Lt) - m) * NS;
-        for (int l0 = m; l0 <= L; l0 += 4) {
+        for (int l0 = m; l0 <= L + 1; l0 += 4) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int l = l0 + i;
                 double cv[NS];
 #pragma unroll
-                for (int c = 0; c < NS; ++c) cv[c] = l <= L ? cf[(int64_t)l * NS + c] : 0.0;
+                for (int c = 0; c < NS; ++c) cv[c] = l <= L + 1 ? cf[(int64_t)l * NS + c] : 0.0;
                 const double a = al[l + 1];
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
@@ -716,12 +737,12 @@ __global__ __launch_bounds__(64 * NW) void k_bat_syn64(const double* __restrict_
         const double rs = 1.0 / (g.sin * g.sin);
 #pragma unroll
         for (int f = 0; f < K2; ++f) {
-            const int b = C::syn_base2 + 8 * f;
+            const int b = C::syn_base2 + 4 * f;
             double nq[4], sq[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
-                nq[c] = (acc[t][0][b + c] + acc[t][1][b + c]) * rs + (acc[t][0][b + 4 + c] + acc[t][1][b + 4 + c]);
-                sq[c] = (acc[t][0][b + c] - acc[t][1][b + c]) * rs + (acc[t][0][b + 4 + c] - acc[t][1][b + 4 + c]);
+                nq[c] = (acc[t][0][b + c] + acc[t][1][b + c]) * rs;
+                sq[c] = (acc[t][0][b + c] - acc[t][1][b + c]) * rs;
             }
             put_spin2(spec2 + f * smap2, ld2, L, m, r, nq[0], nq[1], nq[2], nq[3]);
             if (j > 0) put_spin2(spec2 + f * smap2, ld2, L, m, rp, sq[0], sq[1], sq[2], sq[3]);
@@ -791,7 +812,7 @@ extern "C" int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t 
     const int64_t nring = 4 * nside - 1;
     if (k0 > 1 && smap0 < nring * ld0) return fail(GM_E_SHAPE, "%s: spin-0 map stride too small", name);
     if (k2 > 1 && smap2 < nring * ld2) return fail(GM_E_SHAPE, "%s: spin-2 field stride too small", name);
-    const int64_t tsize = table_size(L + 1);
+    const int64_t tsize = table_size(L + kTabExtra);
     hipStream_t st = (hipStream_t)stream;
     const auto* s0 = reinterpret_cast<const double2*>(spec0);
     const auto* s2 = reinterpret_cast<const double2*>(spec2);
@@ -805,7 +826,7 @@ extern "C" int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t 
             hipLaunchKernelGGL((k_bat_ana<K0, K2, kAna64NW, 1, double>), dim3(nb64, (unsigned)mpol),
                                dim3(64 * kAna64NW), 0, st, s0, ld0, smap0, s2, ld2, smap2, (int)nside, (int)L, alpha,
                                seed_mant, seed_exp, nactive, (int)jsplit, (int)(2 * nside), part64,
-                               table_base(mpol, L + 1));
+                               table_base(mpol, L + kTabExtra));
     })
     return check_launch(name);
 }
@@ -823,8 +844,8 @@ extern "C" int gm_bat_finish(const float* part32, const double* part64, int64_t 
     const int64_t L = lmax + 1, nelem = L * (L + 1) / 2;
     if ((k0 > 1 && astride0 < nelem) || (k2 && astride2 < nelem)) return fail(GM_E_SHAPE, "%s: alm stride too small", name);
     // orders m >= mpol have no ring pair beyond jsplit (their nactive <= jsplit): no FP64 partials
-    BatBlocks bb{kAnaCH32, (int)jsplit, kAnaCH64, (int)(2 * nside), table_base(mpol, L + 1)};
-    const int64_t tsize = table_size(L + 1);
+    BatBlocks bb{kAnaCH32, (int)jsplit, kAnaCH64, (int)(2 * nside), table_base(mpol, L + kTabExtra)};
+    const int64_t tsize = table_size(L + kTabExtra);
     hipStream_t st = (hipStream_t)stream;
     auto* a0 = reinterpret_cast<double2*>(alm0);
     auto* a2 = reinterpret_cast<double2*>(alm2);
@@ -847,7 +868,7 @@ extern "C" int gm_bat_prep(const double* alm0, int64_t k0, int64_t astride0, con
     const auto* a0 = reinterpret_cast<const double2*>(alm0);
     const auto* a2 = reinterpret_cast<const double2*>(alm2);
     GM_BAT_DISPATCH(K0, K2, {
-        hipLaunchKernelGGL((k_bat_prep<K0, K2>), dim3((unsigned)((L + 1 + GM_TABLE_PAD + 255) / 256), (unsigned)L),
+        hipLaunchKernelGGL((k_bat_prep<K0, K2>), dim3((unsigned)((L + kTabExtra + GM_TABLE_PAD + 255) / 256), (unsigned)L),
                            dim3(256), 0, st, a0, astride0, a2, astride2, P, (int)L, coef);
     })
     return check_launch(name);

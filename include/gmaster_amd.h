@@ -161,11 +161,11 @@ int gm_coupled_cell(const double* alm1, int64_t n1, const double* alm2, int64_t 
 /* ---- batched latitudinal pass: K0 real spin-0 maps and K2 spin-2 fields sharing one
  * spin-0 recurrence lambda_lm per (m, ring pair), contraction on the matrix unit.
  * Supported (K0, K2): (2, 1), (1, 1), (0, 1), (0, 2), (2, 0), (4, 0), (8, 0).  Tables
- * (alpha, P, seeds) are the spin-0 ones of lmax + 1 (gm_lat_tables(lmax + 1, 0, ...)): the
- * spin-2 rows need lambda up to degree lmax + 1.  nactive[m] counts ring pairs (equator
+ * (alpha, P, seeds) are the spin-0 ones of lmax + 2 (gm_lat_tables(lmax + 2, 0, ...)): the
+ * spin-2 rows need lambda up to degree lmax + 1, the folded synthesis up to lmax + 2.  nactive[m] counts ring pairs (equator
  * first); pairs j >= jsplit (near the poles) contract in FP64, the others in FP32.
- * The kernels read whole 32-degree blocks: alpha must be followed by 64 zero doubles and
- * the synthesis coefficients by 64 spare degrees (64 x gm_bat_stride(.., 1) doubles).
+ * The kernels read whole blocks of degrees: alpha must be followed by 128 zero doubles and
+ * the synthesis coefficients by 96 spare degrees (96 x gm_bat_stride(.., 1) doubles).
  * Replaces, for this batched form, the march analysis/synthesis FFI handlers
  * gm_march_ana_s0_pair / gm_march_ana_s2 / gm_march_syn_s0_pair / gm_march_syn_s2
  * (gmaster/_native/cuda/march_v2.cu:496-594), which run one recurrence per spin. */
@@ -176,7 +176,7 @@ int64_t gm_bat_blocks(int64_t nside, int64_t jsplit, int64_t polar);
 int64_t gm_bat_stride(int64_t k0, int64_t k2, int64_t synthesis);
 /* Analysis partial sums: spec0 (k0, nring, ld0 >= lmax+1) phased spectra of real maps,
  * spec2 (k2, nring, ld2 >= 2 lmax + 1) phased spectra F+ of Q + iU (column lmax + m);
- * part32: gm_bat_blocks(.., 0) x gm_table_size(lmax + 1) x stride floats; part64, the
+ * part32: gm_bat_blocks(.., 0) x gm_table_size(lmax + 2) x stride floats; part64, the
  * polar blocks, gm_bat_blocks(.., 1) x T x stride doubles with T the table entries of the
  * orders m < mpol (every order with nactive[m] > jsplit). */
 int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, const double* spec2, int64_t k2,
@@ -188,7 +188,7 @@ int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, cons
 int gm_bat_finish(const float* part32, const double* part64, int64_t k0, int64_t k2, int64_t nside, int64_t lmax,
                   const int32_t* nactive, int64_t jsplit, int64_t mpol, const double* P, double* alm0,
                   int64_t astride0, double* alm2, int64_t astride2, int mode, void* stream);
-/* Synthesis coefficients (gm_table_size(lmax + 1) x gm_bat_stride(.., 1) doubles). */
+/* Synthesis coefficients (gm_table_size(lmax + 2) x gm_bat_stride(.., 1) doubles). */
 int gm_bat_prep(const double* alm0, int64_t k0, int64_t astride0, const double* alm2, int64_t k2, int64_t astride2,
                 const double* P, int64_t lmax, double* coef, void* stream);
 /* Synthesis of every ring pair into spec0 / spec2 (layouts as for gm_bat_ana). */

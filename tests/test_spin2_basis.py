@@ -11,7 +11,8 @@ spin-2 rows of orders +m and -m), the HEALPix functions of Gorski et al. (2005) 
 after the recurrence x lambda_{l-1} = eps_l lambda_l + eps_{l-1} lambda_{l-2} removes the
 cos(theta) factors.  So the analysis needs only S0_j = sum_r lambda_j G_r and
 S1_j = sum_r lambda_j G_r / s_r^2 of the Q and U ring spectra, and the synthesis only
-sums of lambda_j times per-degree coefficients, split into a 1/s^2 part and a plain part.
+sums of lambda_j times per-degree coefficients, split into a 1/s^2 part and a plain part;
+the plain part folds into the 1/s^2 part (one coefficient per degree, up to degree lmax + 2).
 """
 
 import numpy as np
@@ -122,10 +123,34 @@ def test_spin2_analysis_and_synthesis_in_the_spin0_basis():
         aU = -(aW[j] * Bm[j] + at(cW, j + 2) * at(Bm, j + 2)
                - 1j * (at(dX, j - 1) * at(Em, j - 1) + at(eX, j + 1) * at(Em, j + 1)))
         bU = -bW[j] * Bm[j]
-        lam = lambda_rows(L + 1, m, 0, R.z, R.shalf, R.chalf)[m:]
-        Q = (aQ @ lam) / s2 + bQ @ lam
-        U = (aU @ lam) / s2 + bU @ lam
+        lam = lambda_rows(L + 3, m, 0, R.z, R.shalf, R.chalf)[m:]  # degrees m .. L + 2
+        Q = (aQ @ lam[:len(j)]) / s2 + bQ @ lam[:len(j)]
+        U = (aU @ lam[:len(j)]) / s2 + bU @ lam[:len(j)]
         sc = max(np.max(np.abs(Fref)), 1e-300)
         assert np.max(np.abs(Q + 1j * U - Fref[:, L - 1 + m])) < 1e-12 * sc, m
         if m:
             assert np.max(np.abs(np.conj(Q) + 1j * np.conj(U) - Fref[:, L - 1 - m])) < 1e-12 * sc, m
+        # the plain part folded into the 1/s^2 part: s^2 lambda_j = (1 - eps_{j+1}^2 - eps_j^2) lambda_j
+        # - eps_{j+1} eps_{j+2} lambda_{j+2} - eps_j eps_{j-1} lambda_{j-2}, one coefficient per degree
+        ee = np.array([np.sqrt(max(k * k - m * m, 0) / (4.0 * k * k - 1.0)) if k >= m and k > 0 else 0.0
+                       for k in range(L + 6)])
+        jj = np.arange(m, L + 2)  # the folded plain part reaches degree lmax + 2 = L + 1
+        bq_full = np.zeros(L + 6, dtype=np.complex128)
+        bu_full = np.zeros(L + 6, dtype=np.complex128)
+        bq_full[j] = bQ
+        bu_full[j] = bU
+
+        def fold(bf):
+            lower = np.where(jj >= 2, bf[np.maximum(jj - 2, 0)], 0)
+            return bf[jj] * (1 - ee[jj + 1] ** 2 - ee[jj] ** 2) - lower * ee[jj - 1] * ee[jj] \
+                - bf[jj + 2] * ee[jj + 1] * ee[jj + 2]
+
+        aq_full = np.zeros(L + 6, dtype=np.complex128)
+        au_full = np.zeros(L + 6, dtype=np.complex128)
+        aq_full[j] = aQ
+        au_full[j] = aU
+        gq = aq_full[jj] + fold(bq_full)
+        gu = au_full[jj] + fold(bu_full)
+        Qf = (gq @ lam[:len(jj)]) / s2
+        Uf = (gu @ lam[:len(jj)]) / s2
+        assert np.max(np.abs(Qf + 1j * Uf - Fref[:, L - 1 + m])) < 1e-12 * sc, m
