@@ -46,6 +46,7 @@ SIGNATURES = {
     "gm_bat_blocks": [_i64, _i64, _i64],
     "gm_bat_stride": [_i64, _i64, _i64],
     "gm_bat_ana": [_p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _i64, _i64, _p, _p, _p],
+    "gm_bat_ana_res": [_p, _i64, _i64, _i64, _p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _i64, _i64, _p, _p, _p],
     "gm_bat_finish": [_p, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _p, _i64, _i32, _p],
     "gm_bat_prep": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _p],
     "gm_bat_syn": [_p, _i64, _i64, _i64, _i64, _p, _p, _p, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p],

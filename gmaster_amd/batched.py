@@ -111,10 +111,11 @@ def _check(k0, k2):
         raise ValueError(f"unsupported batch: {k0} spin-0 maps and {k2} spin-2 fields")
 
 
-def analysis(bp, spec0, spec2, alm0, alm2, mode):
+def analysis(bp, spec0, spec2, alm0, alm2, mode, residual=False):
     """Spectra -> alms (mode 0 writes, 1 subtracts, 2 adds).  ``spec0`` ``(K0, nring, L)``,
     ``spec2`` ``(K2, nring, 2L-1)`` (F+ of Q + iU), ``alm0`` ``(K0, nelem)``, ``alm2``
-    ``(K2, 2, nelem)``; absent kinds are None."""
+    ``(K2, 2, nelem)``; absent kinds are None.  ``residual=True`` (refinement passes, whose
+    input is a residual map) contracts with FP16 hi/lo operands (products to ~2^-21)."""
     k0 = 0 if spec0 is None else spec0.shape[0]
     k2 = 0 if spec2 is None else spec2.shape[0]
     _check(k0, k2)
@@ -124,7 +125,7 @@ def analysis(bp, spec0, spec2, alm0, alm2, mode):
     part64 = (torch.empty(bp.nb64 * bp.tsize64 * S, dtype=torch.float64, device=bp.device)
               if bp.nb64 and bp.mpol else None)
     t = bp.tab
-    _lib.call("gm_bat_ana", _lib.ptr(spec0), k0, spec0.shape[2] if k0 else 0, spec0.stride(0) if k0 else 0,
+    _lib.call("gm_bat_ana_res" if residual else "gm_bat_ana", _lib.ptr(spec0), k0, spec0.shape[2] if k0 else 0, spec0.stride(0) if k0 else 0,
               _lib.ptr(spec2), k2, spec2.shape[2] if k2 else 0, spec2.stride(0) if k2 else 0, bp.nside, bp.lmax,
               _lib.ptr(bp.alpha), _lib.ptr(t.smant), _lib.ptr(t.sexp), _lib.ptr(bp.nact), bp.jsplit, bp.mpol,
               _lib.ptr(part32), _lib.ptr(part64), st)
@@ -218,7 +219,7 @@ def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True):
     d0, d2 = _like(alm0), _like(alm2)
     sign = 1
     for it in range(int(n_iter)):
-        analysis(bp, ra0, ra2, d0, d2, 0)
+        analysis(bp, ra0, ra2, d0, d2, 0, residual=True)
         for a, d in ((alm0, d0), (alm2, d2)):
             if a is not None:
                 if sign > 0:

@@ -271,12 +271,8 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
                         if (i & 1) mu0[t] = fma(x[t] * a[i], mu1[t], -mu0[t]);
                         else mu1[t] = fma(x[t] * a[i], mu0[t], -mu1[t]);
                     }
-                    if (inact[t]) {
-                        if (kx[t] < 0 && fabs(mu0[t]) > ExtExp<double>::inactive_max) {
-                            mu0[t] *= ExtExp<double>::rescale;
-                            mu1[t] *= ExtExp<double>::rescale;
-                            kx[t] += 1;
-                        }
+                    if ((i4 & 1) && inact[t]) {  // every 8 degrees
+                        rescale_lane(mu0[t], mu1[t], kx[t]);
                         inact[t] = __any(kx[t] < 0);
                     }
                 }
@@ -316,6 +312,185 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
             F sum = 0;
 #pragma unroll
             for (int ww = 0; ww < NW; ++ww) sum += tb[ww][u];
+            part[(int64_t)ls * SLOT + u] = sum;
+        }
+        __syncthreads();  // the next super-tile overwrites the buffers
+    }
+}
+
+// ------------------------------------------------------------------ residual analysis
+// The refinement passes analyse residual maps (map minus the synthesis of the current
+// alms), a small fraction of the map, so their contraction can trade precision for rate:
+// both operands are split into FP16 hi + lo parts (hi = the top 11 significand bits of
+// the FP32 value, lo = the remainder rounded to FP16: together ~2^-22 relative) and
+// v_mfma_f32_16x16x32_f16, at 16x the FP32 matrix rate, forms hi*hi + hi*lo + lo*hi in
+// FP32 accumulators (each product to ~2^-21).  FP16's range is met by exact powers of
+// two: mu is carried times 2^9 (|mu| < 2^5 on every ring outside the FP64 polar caps, so
+// |2^9 mu| < 2^14), and each wave scales every channel so that its largest value over
+// the wave's 64 ring pairs lies in [2^14, 2^15); the scales are divided out before the
+// cross-wave sum.  Lane ln runs ring pair js + ln and writes hi and lo into two FP16
+// transpose planes (row = parity, degree / 2); MFMA lane (n, kg) then reads the 8 ring
+// pairs 32 b + 8 kg .. + 7 of degree row n of each plane with one 16-byte load.  Ring
+// blocks, partials and their layout are those of k_bat_ana with R = 1.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+constexpr int kRowH = 64 + 8;      // FP16 per transpose row: 64 ring pairs + 16 bytes of padding
+constexpr double kMuScale = 512.0;  // 2^9
+
+// v = hi + lo: hi keeps the top 11 significand bits (exact in FP16 for |v| >= 2^-14),
+// lo = v - hi (exact in FP32) rounded to FP16; packed as (hi, lo).
+__device__ inline h2 split16(float v) {
+    const float hf = __uint_as_float(__float_as_uint(v) & 0xFFFFE000u);
+    return h2{(_Float16)hf, (_Float16)(v - hf)};
+}
+
+// The same split as one packed word (hi in bits 0-15, lo in 16-31) from one v_cvt_pk_f16_f32
+// (round to nearest even); written out so that the two halves, stored apart, still come
+// from a single convert.
+__device__ inline unsigned split16_packed(float v) {
+    const float hf = __uint_as_float(__float_as_uint(v) & 0xFFFFE000u);
+    const float r = v - hf;
+    unsigned out;
+    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(out) : "v"(hf), "v"(r));
+    return out;
+}
+
+template <int K0, int K2, int NW>
+__global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restrict__ spec0, int64_t ld0, int64_t smap0,
+                                                         const double2* __restrict__ spec2, int64_t ld2, int64_t smap2,
+                                                         int nside, int L, const double* __restrict__ alpha,
+                                                         const double* __restrict__ smant, const int* __restrict__ sexp,
+                                                         const int* __restrict__ nact, int jbeg, int jend,
+                                                         float* __restrict__ partial, int64_t tsize) {
+    typedef Chan<K0, K2> C;
+    constexpr int T = 64 * NW;
+    constexpr int SLOT = C::ana_stride;
+    constexpr int PLANE = 2 * 16 * kRowH;  // FP16 per plane: parity x 16 degree rows x row
+    __shared__ __align__(32) _Float16 tb[NW][2 * PLANE];
+    const int Lt = L + kTabExtra;
+    const int m = blockIdx.y;
+    const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int jhi = min(nact[m], jend);
+    const int j0 = jbeg + blockIdx.x * T;
+    if (j0 >= jhi) return;  // uniform over the workgroup
+    const int js = j0 + w * 64;
+    const bool live = js < jhi;  // wave-uniform
+    const int kg = ln >> 4, ich = ln & 15;
+
+    double x = 0.0, mu0 = 0.0, mu1 = 0.0;
+    int kx = 0;
+    if (js + ln < jhi) {
+        const Ring g = ring_info(nside, 2 * nside - 1 - (js + ln));
+        x = g.z;
+        seed_lane<double>(smant[m], sexp[m], m, m, g.shalf, g.chalf, mu0, kx);
+        mu0 *= kMuScale;
+    }
+    bool inact = __any(kx < 0);
+
+    // A operands: channel ich of ring pairs js + 32 b + 8 kg + u (u = 0..7), parity p
+    h8 ah[2][2], al[2][2];
+    float inv[4];  // 1 / (2^9 x scale) of accumulator rows 4 kg + v
+    {
+        double ev[2][8], ov[2][8];
+        double mx = 0.0;
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int jq = js + 32 * b + 8 * kg + u;
+                double e = 0.0, o = 0.0;
+                if (jq < jhi && ich < C::nc) chan_eo<K0, K2>(spec0, ld0, smap0, spec2, ld2, smap2, nside, L, m, jq, ich, e, o);
+                ev[b][u] = e;
+                ov[b][u] = o;
+                mx = fmax(mx, fmax(fabs(e), fabs(o)));
+            }
+        mx = fmax(mx, __shfl_xor(mx, 16));
+        mx = fmax(mx, __shfl_xor(mx, 32));
+        const int ex = mx > 0.0 ? ilogb(mx) : 0;  // this channel's largest value lies in [2^ex, 2^(ex+1))
+        const double sc = ldexp(1.0, 14 - ex);
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const h2 pe = split16((float)(ev[b][u] * sc)), po = split16((float)(ov[b][u] * sc));
+                ah[0][b][u] = pe[0];
+                al[0][b][u] = pe[1];
+                ah[1][b][u] = po[0];
+                al[1][b][u] = po[1];
+            }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) inv[v] = ldexpf(1.f, __shfl(ex, 4 * kg + v) - 14 - 9);
+    }
+
+    const double* al_ = alpha + table_base(m, Lt) - m;
+    float* part = partial + ((int64_t)blockIdx.x * tsize + table_base(m, Lt) - m) * SLOT;
+    _Float16* hp = tb[w];
+    _Float16* lp = tb[w] + PLANE;
+    for (int ls = m; ls <= L; ls += kSuper * kTile) {
+        f4 acc[kSuper][2];
+#pragma unroll
+        for (int h = 0; h < kSuper; ++h) {
+            const int l0 = ls + h * kTile;
+            double a[kTile];
+#pragma unroll
+            for (int i = 0; i < kTile; ++i) a[i] = al_[l0 + i + 1];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[h][0][v] = acc[h][1][v] = 0.f;
+            if (!live) continue;
+#pragma unroll
+            for (int i4 = 0; i4 < kTile / 4; ++i4) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i = 4 * i4 + u;
+                    const double cur = (i & 1) ? mu1 : mu0;
+                    const unsigned pk = split16_packed((float)cur);
+                    const int at = ((i & 1) * 16 + (i >> 1)) * kRowH + ln;
+                    reinterpret_cast<unsigned short*>(hp)[at] = (unsigned short)pk;
+                    reinterpret_cast<unsigned short*>(lp)[at] = (unsigned short)(pk >> 16);
+                    if (i & 1) mu0 = fma(x * a[i], mu1, -mu0);
+                    else mu1 = fma(x * a[i], mu0, -mu1);
+                }
+                if ((i4 & 1) && inact) {  // every 8 degrees
+                    rescale_lane(mu0, mu1, kx);
+                    inact = __any(kx < 0);
+                }
+            }
+            wave_sync();
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int off = (p * 16 + ich) * kRowH + 32 * b + 8 * kg;
+                    const h8 bh = *reinterpret_cast<const h8*>(hp + off);
+                    const h8 bl = *reinterpret_cast<const h8*>(lp + off);
+                    acc[h][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[p][b], bh, acc[h][p], 0, 0, 0);
+                    acc[h][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[p][b], bl, acc[h][p], 0, 0, 0);
+                    acc[h][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[p][b], bh, acc[h][p], 0, 0, 0);
+                }
+            wave_sync();
+        }
+        // acc[h][p][v] at lane (kg, n): channel 4 kg + v, degree ls + 32 h + 2 n + p; unscaled
+        // into the wave's (now free) buffer in output order, then summed over the waves
+        constexpr int NOUT = kSuper * kTile * SLOT;
+        static_assert(NOUT * sizeof(float) <= 2 * PLANE * sizeof(_Float16), "super-tile must fit the buffer");
+        float* mytb = reinterpret_cast<float*>(tb[w]);
+#pragma unroll
+        for (int h = 0; h < kSuper; ++h)
+#pragma unroll
+            for (int p = 0; p < 2; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = 4 * kg + v;
+                    if (c < SLOT) mytb[(h * kTile + 2 * ich + p) * SLOT + c] = acc[h][p][v] * inv[v];
+                }
+        __syncthreads();
+        for (int u = threadIdx.x; u < NOUT; u += T) {
+            const int l = ls + u / SLOT;
+            if (l > L) continue;
+            float sum = 0.f;
+#pragma unroll
+            for (int ww = 0; ww < NW; ++ww) sum += reinterpret_cast<const float*>(tb[ww])[u];
             part[(int64_t)ls * SLOT + u] = sum;
         }
         __syncthreads();  // the next super-tile overwrites the buffers
@@ -567,6 +742,7 @@ __global__ __launch_bounds__(64 * NW, GM_BAT_SYN_WAVES) void k_bat_syn(const dou
                 my[ch * SR + d] = lb + d <= L + 1 ? (float)pf[v] : 0.f;
             }
             wave_sync();
+#pragma unroll 1
             for (int i4 = 0; i4 < kTile / 4; ++i4) {
                 const int l0 = lb + 4 * i4;
                 if (i4 == kTile / 8) {  // the next block's coefficients, half a block ahead
@@ -601,7 +777,7 @@ __global__ __launch_bounds__(64 * NW, GM_BAT_SYN_WAVES) void k_bat_syn(const dou
 #pragma unroll
                         for (int q = 0; q < NG; ++q) acc[i & 1][t][q] = mfma4(cur[t][i], cb[q][i], acc[i & 1][t][q]);
                 __builtin_amdgcn_sched_barrier(0);
-                rescale<R, double>(mu0, mu1, kx, anyinact);
+                if (i4 & 1) rescale<R, double>(mu0, mu1, kx, anyinact);
             }
         }
     }
@@ -660,8 +836,9 @@ __global__ __launch_bounds__(64 * NW, GM_BAT_SYN_WAVES) void k_bat_syn(const dou
 }
 
 // Polar ring pairs in FP64 on the vector ALU: R ring pairs per lane, every channel in
-// registers (no cross-lane exchange in the epilogue); each degree's coefficients are
-// uniform scalar loads shared by the R ring pairs.
+// registers (no cross-lane exchange in the epilogue).  The coefficients of each block of
+// kTile degrees are staged in a wave-private LDS buffer (one coalesced load, the next
+// block's issued half a block ahead) and read back as broadcasts.
 template <int K0, int K2, int NW, int R>
 __global__ __launch_bounds__(64 * NW) void k_bat_syn64(const double* __restrict__ coef, int nside, int L,
                                                        const double* __restrict__ alpha,
@@ -672,6 +849,9 @@ __global__ __launch_bounds__(64 * NW) void k_bat_syn64(const double* __restrict_
     typedef Chan<K0, K2> C;
     constexpr int NS = C::syn_stride;
     constexpr int T = 64 * NW;
+    constexpr int NV = kTile * NS / 64;  // coefficients each lane stages per block
+    static_assert(kTile * NS % 64 == 0 && NS % 2 == 0, "blocks must split evenly over a wave");
+    __shared__ __align__(16) double cst[NW][kTile * NS];
     const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int jhi = min(nact[m], jend);
@@ -701,24 +881,47 @@ __global__ __launch_bounds__(64 * NW) void k_bat_syn64(const double* __restrict_
         anyinact = __any(anyinact);
         const double* al = alpha + table_base(m, Lt) - m;
         const double* cf = coef + (table_base(m, Lt) - m) * NS;
-        for (int l0 = m; l0 <= L + 1; l0 += 4) {
+        double* my = cst[threadIdx.x >> 6];
+        const int ln = threadIdx.x & 63;
+        double pf[NV];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int l = l0 + i;
-                double cv[NS];
+        for (int v = 0; v < NV; ++v) pf[v] = cf[(int64_t)m * NS + v * 64 + ln];
+        for (int lb = m; lb <= L + 1; lb += kTile) {
+            wave_sync();  // the previous block's reads are done
 #pragma unroll
-                for (int c = 0; c < NS; ++c) cv[c] = l <= L + 1 ? cf[(int64_t)l * NS + c] : 0.0;
-                const double a = al[l + 1];
-#pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    const double cur = (i & 1) ? mu1[t] : mu0[t];
-#pragma unroll
-                    for (int c = 0; c < NS; ++c) acc[t][i & 1][c] = fma(cur, cv[c], acc[t][i & 1][c]);
-                    if (i & 1) mu0[t] = fma(x[t] * a, mu1[t], -mu0[t]);
-                    else mu1[t] = fma(x[t] * a, mu0[t], -mu1[t]);
-                }
+            for (int v = 0; v < NV; ++v) {
+                const int e = v * 64 + ln;  // = d * NS + c
+                my[e] = lb + e / NS <= L + 1 ? pf[v] : 0.0;
             }
-            rescale<R, double>(mu0, mu1, kx, anyinact);
+            wave_sync();
+#pragma unroll 1
+            for (int i4 = 0; i4 < kTile / 4; ++i4) {
+                if (i4 == kTile / 8) {  // the next block's coefficients, half a block ahead
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) pf[v] = cf[(int64_t)(lb + kTile) * NS + v * 64 + ln];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int d = 4 * i4 + i, l = lb + d;
+                    double cv[NS];
+#pragma unroll
+                    for (int c2 = 0; c2 < NS / 2; ++c2) {
+                        const double2 v2 = *reinterpret_cast<const double2*>(my + d * NS + 2 * c2);
+                        cv[2 * c2] = v2.x;
+                        cv[2 * c2 + 1] = v2.y;
+                    }
+                    const double a = al[l + 1];
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const double cur = (i & 1) ? mu1[t] : mu0[t];
+#pragma unroll
+                        for (int c = 0; c < NS; ++c) acc[t][i & 1][c] = fma(cur, cv[c], acc[t][i & 1][c]);
+                        if (i & 1) mu0[t] = fma(x[t] * a, mu1[t], -mu0[t]);
+                        else mu1[t] = fma(x[t] * a, mu0[t], -mu1[t]);
+                    }
+                }
+                if (i4 & 1) rescale<R, double>(mu0, mu1, kx, anyinact);
+            }
         }
     }
 #pragma unroll
@@ -795,11 +998,11 @@ extern "C" int64_t gm_bat_stride(int64_t k0, int64_t k2, int64_t synthesis) {
     return synthesis ? stride_syn(k0, k2) : stride_ana(k0, k2);
 }
 
-extern "C" int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, const double* spec2, int64_t k2,
-                          int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax, const double* alpha,
-                          const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit,
-                          int64_t mpol, float* part32, double* part64, void* stream) {
-    const char* name = "gm_bat_ana";
+namespace {
+int bat_ana(const char* name, bool residual, const double* spec0, int64_t k0, int64_t ld0, int64_t smap0,
+            const double* spec2, int64_t k2, int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax,
+            const double* alpha, const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive,
+            int64_t jsplit, int64_t mpol, float* part32, double* part64, void* stream) {
     if (int e = check_bat(name, k0, k2, nside, lmax, jsplit)) return e;
     if (mpol < 0 || mpol > lmax + 1) return fail(GM_E_ARG, "%s: mpol outside [0, lmax + 1]", name);
     if ((k0 && !spec0) || (k2 && !spec2) || !alpha || !seed_mant || !seed_exp || !nactive)
@@ -817,8 +1020,13 @@ extern "C" int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t 
     const auto* s0 = reinterpret_cast<const double2*>(spec0);
     const auto* s2 = reinterpret_cast<const double2*>(spec2);
     const int nb32 = (int)gm_bat_blocks(nside, jsplit, 0), nb64 = (int)gm_bat_blocks(nside, jsplit, 1);
+    static_assert(kAnaCH32 % 64 == 0 && kAnaCH32 <= 1024, "residual kernel: one ring pair per lane, same blocks");
     GM_BAT_DISPATCH(K0, K2, {
-        if (nb32 > 0)
+        if (nb32 > 0 && residual)
+            hipLaunchKernelGGL((k_bat_ana_h<K0, K2, kAnaCH32 / 64>), dim3(nb32, (unsigned)L), dim3(kAnaCH32), 0, st, s0,
+                               ld0, smap0, s2, ld2, smap2, (int)nside, (int)L, alpha, seed_mant, seed_exp, nactive, 0,
+                               (int)jsplit, part32, tsize);
+        else if (nb32 > 0)
             hipLaunchKernelGGL((k_bat_ana<K0, K2, kAnaNW, kAnaR, float>), dim3(nb32, (unsigned)L), dim3(64 * kAnaNW), 0,
                                st, s0, ld0, smap0, s2, ld2, smap2, (int)nside, (int)L, alpha, seed_mant, seed_exp,
                                nactive, 0, (int)jsplit, part32, tsize);
@@ -829,6 +1037,23 @@ extern "C" int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t 
                                table_base(mpol, L + kTabExtra));
     })
     return check_launch(name);
+}
+}  // namespace
+
+extern "C" int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, const double* spec2, int64_t k2,
+                          int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax, const double* alpha,
+                          const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit,
+                          int64_t mpol, float* part32, double* part64, void* stream) {
+    return bat_ana("gm_bat_ana", false, spec0, k0, ld0, smap0, spec2, k2, ld2, smap2, nside, lmax, alpha, seed_mant,
+                   seed_exp, nactive, jsplit, mpol, part32, part64, stream);
+}
+
+extern "C" int gm_bat_ana_res(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, const double* spec2,
+                              int64_t k2, int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax, const double* alpha,
+                              const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit,
+                              int64_t mpol, float* part32, double* part64, void* stream) {
+    return bat_ana("gm_bat_ana_res", true, spec0, k0, ld0, smap0, spec2, k2, ld2, smap2, nside, lmax, alpha, seed_mant,
+                   seed_exp, nactive, jsplit, mpol, part32, part64, stream);
 }
 
 extern "C" int gm_bat_finish(const float* part32, const double* part64, int64_t k0, int64_t k2, int64_t nside,

@@ -42,17 +42,29 @@ __device__ inline void seed_lane(double smant, int sexp, int a, int b, double sh
     }
 }
 
+// One rescaling step of an inactive lane (k < 0) whose |mu| has passed 2^inactive: a
+// select of the factor and two multiplies (exact: the result stays a normal number).
+template <typename F>
+__device__ inline void rescale_lane(F& mu0, F& mu1, int& k) {
+    const bool up = k < 0 && fabs(mu0) > ExtExp<F>::inactive_max;
+    const F f = up ? ExtExp<F>::rescale : F(1);
+    mu0 *= f;
+    mu1 *= f;
+    k += up ? 1 : 0;
+}
+
+// Rescaling of R lanes' recurrences; anyinact (wave-uniform) skips it once every lane is
+// active.  Callers check every 4 degrees, or every 8 where m <= 12288: over 8 degrees from
+// l0 > m a lane grows by at most prod_{k=1..8} (1 + alpha_{m+k}) < 2^55 (alpha_{m+k}^2 <
+// 4 (m + k) / k), so an inactive FP64 lane found just above 2^-300 had stayed below
+// 2^-245: an FP32 zero, and negligible in FP64.
 template <int R, typename F>
 __device__ inline void rescale(F (&mu0)[R], F (&mu1)[R], int (&kx)[R], bool& anyinact) {
     if (!anyinact) return;
     bool still = false;
 #pragma unroll
     for (int t = 0; t < R; ++t) {
-        if (kx[t] < 0 && fabs(mu0[t]) > ExtExp<F>::inactive_max) {
-            mu0[t] *= ExtExp<F>::rescale;
-            mu1[t] *= ExtExp<F>::rescale;
-            kx[t] += 1;
-        }
+        rescale_lane(mu0[t], mu1[t], kx[t]);
         still = still || (kx[t] < 0);
     }
     anyinact = __any(still);

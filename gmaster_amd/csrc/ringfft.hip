@@ -107,22 +107,61 @@ __device__ void radix_pass(double2* buf, int N, int Ns, double sign) {
     __syncthreads();
 }
 
-// Direct DFT pass of any radix p: each thread computes whole outputs by Horner's rule
-// (at most 8 outputs per thread with blockDim.x >= N / 8).
-__device__ void dft_pass(double2* buf, int N, int p, int Ns, double sign) {
+// Direct DFT pass of any radix p (a prime above 5): output f = j + q J (J = N / p) is
+// sum_u x[j + u J] w_f^u, w_f = e^{sign 2 pi i (j mod Ns + q Ns) / (Ns p)}, by Horner's rule.
+// Each thread takes one residue j and up to 8 values of q (q = qb + t Tj with Tj = T / J
+// threads per residue), so its 8 Horner chains read the same input: one LDS read per step
+// feeds 8 independent complex multiply-adds.  When 8 chains per thread do not cover p
+// (T < J p / 8 after rounding) each thread computes outputs f = tid + t T instead, one
+// read per chain and step.
+// a w + x with two fused multiply-adds per component
+__device__ inline double2 horner_step(double2 a, double2 w, double2 x) {
+    return make_double2(fma(a.x, w.x, fma(-a.y, w.y, x.x)), fma(a.x, w.y, fma(a.y, w.x, x.y)));
+}
+
+__device__ __forceinline__ void dft_pass(double2* buf, int N, int p, int Ns, double sign) {
     const int J = N / p;
     const int T = blockDim.x;
-    double2 out[8];
+    const int Tj = T / J;
+    double2 w[8], acc[8];
+    if (Tj > 0 && 8 * Tj >= p) {
+        const int j = threadIdx.x % J, qb = threadIdx.x / J;
+        const bool active = qb < Tj;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int q = min(qb + t * Tj, p - 1);
+            w[t] = twiddle(j % Ns + (int64_t)q * Ns, (int64_t)Ns * p, sign);
+        }
+        if (active) {
+            const double2 x0 = buf[j + (p - 1) * J];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) acc[t] = x0;
+            for (int u = p - 2; u >= 0; --u) {
+                const double2 x = buf[j + u * J];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) acc[t] = horner_step(acc[t], w[t], x);
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int q = qb + t * Tj;
+                if (q < p) buf[(j / Ns) * Ns * p + j % Ns + q * Ns] = acc[t];
+            }
+        }
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
         const int f = threadIdx.x + t * T;
         if (f < N) {
             const int j = f % J, q = f / J;
-            const int k = j % Ns;
-            const double2 w = twiddle(k + (int64_t)q * Ns, (int64_t)Ns * p, sign);
-            double2 acc = buf[j + (p - 1) * J];
-            for (int u = p - 2; u >= 0; --u) acc = cadd(cmul(acc, w), buf[j + u * J]);
-            out[t] = acc;
+            const double2 wf = twiddle(j % Ns + (int64_t)q * Ns, (int64_t)Ns * p, sign);
+            double2 a = buf[j + (p - 1) * J];
+            for (int u = p - 2; u >= 0; --u) a = horner_step(a, wf, buf[j + u * J]);
+            acc[t] = a;
         }
     }
     __syncthreads();
@@ -131,8 +170,7 @@ __device__ void dft_pass(double2* buf, int N, int p, int Ns, double sign) {
         const int f = threadIdx.x + t * T;
         if (f < N) {
             const int j = f % J, q = f / J;
-            const int k = j % Ns;
-            buf[(j / Ns) * Ns * p + k + q * Ns] = out[t];
+            buf[(j / Ns) * Ns * p + j % Ns + q * Ns] = acc[t];
         }
     }
     __syncthreads();
@@ -146,8 +184,12 @@ __device__ int smallest_factor(int n) {
 }
 
 // In-place complex FFT of length N in LDS, sign -1 forward, +1 inverse (unnormalised).
-// Requires blockDim.x >= N / 8 (at most 8 outputs per thread per pass).
-__device__ void fft_lds(double2* buf, int N, double sign) {
+// Requires blockDim.x >= N / 8 (at most 8 outputs per thread per pass).  Inlined into each
+// kernel, so the library makes no device-side calls: compiled as a called function with
+// the FMA Horner step above, it returned spectra off by up to ~1e-7 relative on gfx950
+// (rings that never reach the direct pass included), while the inlined form matches
+// numpy to ~1e-16; the cause was not isolated.
+__device__ __forceinline__ void fft_lds(double2* buf, int N, double sign) {
     int Ns = 1, rem = N;
     while (rem > 1) {
         int p;

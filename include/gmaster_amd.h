@@ -183,6 +183,15 @@ int gm_bat_ana(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, cons
                int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax, const double* alpha, const double* seed_mant,
                const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit, int64_t mpol, float* part32,
                double* part64, void* stream);
+/* gm_bat_ana for residual maps (the Jacobi refinement passes): the FP32 ring blocks contract
+ * with FP16 hi + lo operands on the matrix unit (each product to ~2^-21 relative, ~4x the
+ * per-term error of gm_bat_ana, on an input that is a small fraction of the map); same
+ * arguments, partials and polar FP64 blocks.  Replaces, in the refinement loop of the
+ * batched form, the same march analysis handlers as gm_bat_ana. */
+int gm_bat_ana_res(const double* spec0, int64_t k0, int64_t ld0, int64_t smap0, const double* spec2, int64_t k2,
+                   int64_t ld2, int64_t smap2, int64_t nside, int64_t lmax, const double* alpha,
+                   const double* seed_mant, const int32_t* seed_exp, const int32_t* nactive, int64_t jsplit,
+                   int64_t mpol, float* part32, double* part64, void* stream);
 /* Sums the partials, applies P_l and forms spin-0 alms (alm0 + k astride0) and E, B
  * (alm2 + (2 f + c) astride2, c = 0 for E, 1 for B); mode 0 writes, 1 subtracts, 2 adds. */
 int gm_bat_finish(const float* part32, const double* part64, int64_t k0, int64_t k2, int64_t nside, int64_t lmax,

@@ -2,8 +2,8 @@
 
 * Ring FFTs of rings above the LDS limit (the belt and the caps with q > 4096 have more
   than 8192 packed points and run as two halves) against numpy's FFT on sampled rings,
-  forward and inverse, spin 0 and spin 2.  The same kernels at small sizes are covered by
-  the ring tests with the LDS limit lowered at build time.
+  forward and inverse, spin 0 and spin 2 (the in-LDS ring kernels are checked ring by ring
+  at Nside 8 to 4096 in test_gpu_ringfft.py).
 * Multi-pass Legendre analyses (the ring chunks are covered four partial slots at a time
   above Nside 4096) against the C oracle on sampled orders, FP64 and mixed.
 * The full T+P MASTER pipeline in the chosen and the plain FP64 form: finite bandpowers
