@@ -328,14 +328,15 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
 // two: mu is carried times 2^9 (|mu| < 2^5 on every ring outside the FP64 polar caps, so
 // |2^9 mu| < 2^14), and each wave scales every channel so that its largest value over
 // the wave's 64 ring pairs lies in [2^14, 2^15); the scales are divided out before the
-// cross-wave sum.  Lane ln runs ring pair js + ln and writes hi and lo into two FP16
-// transpose planes (row = parity, degree / 2); MFMA lane (n, kg) then reads the 8 ring
-// pairs 32 b + 8 kg .. + 7 of degree row n of each plane with one 16-byte load.  Ring
-// blocks, partials and their layout are those of k_bat_ana with R = 1.
+// cross-wave sum.  Lane ln runs ring pair js + ln and writes its (hi, lo) pair as one
+// word into column ln of the transpose tile (row = parity, degree / 2; two degrees per
+// LDS write, as the FP32 kernel); MFMA lane (n, kg) reads the 8 words of ring pairs
+// 32 b + 8 kg .. + 7 in degree row n (two 16-byte loads) and separates hi and lo with
+// byte permutes.  Ring blocks, partials and their layout are those of k_bat_ana, R = 1.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
-constexpr int kRowH = 64 + 8;      // FP16 per transpose row: 64 ring pairs + 16 bytes of padding
+constexpr int kRowH = 64 + 4;      // words per transpose row: 64 ring pairs + 4 words of padding
 constexpr double kMuScale = 512.0;  // 2^9
 
 // v = hi + lo: hi keeps the top 11 significand bits (exact in FP16 for |v| >= 2^-14),
@@ -366,8 +367,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
     typedef Chan<K0, K2> C;
     constexpr int T = 64 * NW;
     constexpr int SLOT = C::ana_stride;
-    constexpr int PLANE = 2 * 16 * kRowH;  // FP16 per plane: parity x 16 degree rows x row
-    __shared__ __align__(32) _Float16 tb[NW][2 * PLANE];
+    constexpr int TILE = 2 * 16 * kRowH;  // words per wave: parity x 16 degree rows x row
+    __shared__ __align__(32) unsigned tb[NW][TILE];
     const int Lt = L + kTabExtra;
     const int m = blockIdx.y;
     const int w = threadIdx.x >> 6, ln = threadIdx.x & 63;
@@ -425,8 +426,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
 
     const double* al_ = alpha + table_base(m, Lt) - m;
     float* part = partial + ((int64_t)blockIdx.x * tsize + table_base(m, Lt) - m) * SLOT;
-    _Float16* hp = tb[w];
-    _Float16* lp = tb[w] + PLANE;
+    unsigned* mytw = tb[w];
     for (int ls = m; ls <= L; ls += kSuper * kTile) {
         f4 acc[kSuper][2];
 #pragma unroll
@@ -445,9 +445,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
                     const int i = 4 * i4 + u;
                     const double cur = (i & 1) ? mu1 : mu0;
                     const unsigned pk = split16_packed((float)cur);
-                    const int at = ((i & 1) * 16 + (i >> 1)) * kRowH + ln;
-                    reinterpret_cast<unsigned short*>(hp)[at] = (unsigned short)pk;
-                    reinterpret_cast<unsigned short*>(lp)[at] = (unsigned short)(pk >> 16);
+                    mytw[((i & 1) * 16 + (i >> 1)) * kRowH + ln] = pk;
                     if (i & 1) mu0 = fma(x * a[i], mu1, -mu0);
                     else mu1 = fma(x * a[i], mu0, -mu1);
                 }
@@ -461,9 +459,20 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
             for (int p = 0; p < 2; ++p)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
-                    const int off = (p * 16 + ich) * kRowH + 32 * b + 8 * kg;
-                    const h8 bh = *reinterpret_cast<const h8*>(hp + off);
-                    const h8 bl = *reinterpret_cast<const h8*>(lp + off);
+                    const unsigned* src = mytw + (p * 16 + ich) * kRowH + 32 * b + 8 * kg;
+                    const uint4 w0 = *reinterpret_cast<const uint4*>(src);
+                    const uint4 w1 = *reinterpret_cast<const uint4*>(src + 4);
+                    // words (hi_u | lo_u << 16) of ring pairs u = 0..7 -> hi halves, lo halves
+                    const unsigned hw[4] = {__builtin_amdgcn_perm(w0.y, w0.x, 0x05040100u),
+                                            __builtin_amdgcn_perm(w0.w, w0.z, 0x05040100u),
+                                            __builtin_amdgcn_perm(w1.y, w1.x, 0x05040100u),
+                                            __builtin_amdgcn_perm(w1.w, w1.z, 0x05040100u)};
+                    const unsigned lw[4] = {__builtin_amdgcn_perm(w0.y, w0.x, 0x07060302u),
+                                            __builtin_amdgcn_perm(w0.w, w0.z, 0x07060302u),
+                                            __builtin_amdgcn_perm(w1.y, w1.x, 0x07060302u),
+                                            __builtin_amdgcn_perm(w1.w, w1.z, 0x07060302u)};
+                    const h8 bh = __builtin_bit_cast(h8, hw);
+                    const h8 bl = __builtin_bit_cast(h8, lw);
                     acc[h][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[p][b], bh, acc[h][p], 0, 0, 0);
                     acc[h][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[p][b], bl, acc[h][p], 0, 0, 0);
                     acc[h][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[p][b], bh, acc[h][p], 0, 0, 0);
@@ -473,7 +482,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
         // acc[h][p][v] at lane (kg, n): channel 4 kg + v, degree ls + 32 h + 2 n + p; unscaled
         // into the wave's (now free) buffer in output order, then summed over the waves
         constexpr int NOUT = kSuper * kTile * SLOT;
-        static_assert(NOUT * sizeof(float) <= 2 * PLANE * sizeof(_Float16), "super-tile must fit the buffer");
+        static_assert(NOUT <= TILE, "super-tile must fit the buffer");
         float* mytb = reinterpret_cast<float*>(tb[w]);
 #pragma unroll
         for (int h = 0; h < kSuper; ++h)
