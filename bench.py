@@ -57,6 +57,17 @@ def legendre_flops(nside, lmax):
     return 2 * (2 + 4) * T0, 2 * (2 + 4) * T2
 
 
+def _n3j(lmax):
+    """l3 terms of the exact coupling: sum over l1 <= l2 of the triangle-rule count."""
+    import numpy as np
+
+    L = lmax + 1
+    l1 = np.arange(L, dtype=np.int64)[:, None]
+    l2 = np.arange(L, dtype=np.int64)[None, :]
+    cnt = np.minimum(l1 + l2, lmax) - (l2 - l1) + 1
+    return int(np.sum(np.where(l2 >= l1, np.maximum(cnt, 0), 0)))
+
+
 def _measured_traffic(nside, name):
     """HBM bytes per launch of kernel ``name`` from the committed rocprofv3 FETCH_SIZE +
     WRITE_SIZE summary at this Nside (copy-kernel scaled), or None when there is none."""
@@ -69,7 +80,7 @@ def _measured_traffic(nside, name):
     return None
 
 
-def run_pipeline(nmt, mask, tqu, nside, nlb, form="fp64", info=None):
+def run_pipeline(nmt, mask, tqu, nside, nlb, form="fp64", info=None, keep=None):
     from gmaster_amd import batched, field, sht, workspaces
 
     nmt.set_form(form)
@@ -87,7 +98,16 @@ def run_pipeline(nmt, mask, tqu, nside, nlb, form="fp64", info=None):
         if info is not None:
             info.append(w.coupling_info)
     nmt.set_form("fp64")
+    if keep is not None:
+        keep["alms"] = {"T": fT.alm[0], "E": fP.alm[0], "B": fP.alm[1], "mask": fT.alm_mask[0]}
     return out
+
+
+def alm_errors(got, ref):
+    """Largest |a - a_fp64| relative to the largest |a_fp64|, per field component."""
+    import torch
+
+    return {k: float(torch.max(torch.abs(got[k] - ref[k])) / torch.max(torch.abs(ref[k]))) for k in ref}
 
 
 def bandpower_errors(bp, ref):
@@ -276,9 +296,11 @@ def main():
         bp = None
         for i in range(args.steps):
             info.clear()
+            keep.pop(form, None)
+            kept = keep.setdefault(form, {}) if i == args.steps - 1 else None
             torch.cuda.synchronize()
             t = time.perf_counter()
-            bp = run_pipeline(nmt, mask, tqu, nside, nlb, form, info)
+            bp = run_pipeline(nmt, mask, tqu, nside, nlb, form, info, kept)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t)
             log(f"{form} step {i}: {times[-1]:.3f}s")
@@ -287,13 +309,18 @@ def main():
         return times, bp, timers, list(info)
 
     forms = ["fp64"] if args.form == "fp64" else ["fp64", args.form]
+    keep = {}
     runs = {f: timed(f, True) for f in forms}
     times_fp64, bp_fp64, timers_fp64, _ = runs["fp64"]
     times, bp, timers, coupling_info = runs[args.form]
     total = sum(times)
     bp_err = bandpower_errors(bp, bp_fp64) if args.form != "fp64" else None
+    alm_err = None
     if bp_err is not None:
         log(f"{args.form} vs fp64 bandpower errors: {bp_err}")
+        alm_err = alm_errors(keep[args.form]["alms"], keep["fp64"]["alms"])
+        log(f"{args.form} vs fp64 alm errors (max |da| / max |a|): {alm_err}")
+    keep.clear()
 
     def per_call(tm):
         # per entry point: mean duration per launch (ms) over the timed steps
@@ -353,10 +380,63 @@ def main():
             "mean_launch_ms": dom_ms,
         }
 
-    if "gm_bat_ana" in stages:
+    L = lmax + 1
+    T0 = L * (L + 1) / 2 * 2 * nside  # (l, m, folded ring pair) triples of a batched pass
+    nring = 4 * nside - 1
+    npix = 12 * nside * nside
+    PEAK_F16 = 2.5e15  # MI355X dense FP16 matrix, FLOP/s
+    PEAK_HBM = 8.0e12
+
+    def hbm_entry(name, kernel, nbytes, what):
+        ms = stages.get(name, {}).get("mean_ms", float("nan"))
+        ach = nbytes / (ms * 1e-3)
+        return {"kernel": kernel, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "bytes_per_launch": nbytes, "bytes": what, "mean_launch_ms": ms}
+
+    def flop_entry(name, kernel, flop, peak, pipe, st=None):
+        ms = (st or stages).get(name, {}).get("mean_ms", float("nan"))
+        ach = flop / (ms * 1e-3)
+        return {"kernel": kernel, "bound": pipe, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                "frac": ach / peak, "flop_per_launch": flop, "mean_launch_ms": ms}
+
+    def roof_residual(st):
+        # the refinement passes' analysis (the largest share of the step): per (l, m, ring
+        # pair) 2 FP64 FMA of the recurrence + 12 channel FMA, the latter as 3 FP16 hi/lo
+        # products on the matrix unit; op-mix peak of FP64 VALU and FP16 MFMA
+        e = flop_entry("gm_bat_ana_res", "k_bat_ana_h (residual T + mask + Q/U analysis, gm_bat_ana_res: FP16 "
+                       "hi/lo ring blocks on the matrix unit + FP64 polar blocks)", 2 * (2 + 12) * T0,
+                       28.0 / (4.0 / PEAK_FP64 + 3 * 24.0 / PEAK_F16), "fp64 valu + f16 mfma (3 hi/lo products)", st)
+        e["pipe"] = ("FP64 VALU recurrence (78.6 TFLOP/s) + FP16 MFMA contraction (2.5 PFLOP/s, 3 products per "
+                     "FP32 product): op-mix peak; the FP16 split (4 VALU ops per mu) is not in the FLOP count")
+        e["traffic"] = _measured_traffic(nside, "k_bat_ana_h")
+        return e
+
+    if "gm_bat_ana_res" in stages:
+        roofline = roof_residual(stages)
+    elif "gm_bat_ana" in stages:
         roofline = roof_batched(stages)
     else:
         roofline = roof(stages, args.form != "fp64")
+    stage_roofs = {}
+    if "gm_bat_ana" in stages:
+        stage_roofs["first_analysis"] = roof_batched(stages)
+        stage_roofs["synthesis"] = flop_entry(
+            "gm_bat_syn", "k_bat_syn (T + mask + Q/U synthesis, gm_bat_syn: FP32 4x4x1 MFMA + FP64 polar pairs)",
+            2 * (2 + 8) * T0, 20.0 / (4.0 / PEAK_FP64 + 16.0 / PEAK_FP32), "fp64 valu + fp32 mfma")
+        spec_b = nring * L * 16
+        stage_roofs["fft_fwd_t_mask"] = hbm_entry(
+            "gm_ring_fft_fwd_s0", "k_ring_fwd_s0 (T and mask)", 2 * (npix * 8 + spec_b), "2 maps read + 2 spectra written")
+        stage_roofs["fft_fwd_qu"] = hbm_entry(
+            "gm_ring_fft_fwd_s", "k_ring_fwd_s (Q + iU)", 2 * npix * 8 + nring * (2 * L - 1) * 16,
+            "Q, U read + F+ spectrum written")
+        stage_roofs["fold_t_mask"] = hbm_entry(
+            "gm_ring_fold_residual", "k_fold_s0 (T and mask)", 3 * 2 * spec_b, "synthesis + residual read, residual written")
+        stage_roofs["fold_qu"] = hbm_entry(
+            "gm_ring_fold_residual_c", "k_fold_s (Q + iU)", 3 * nring * (2 * L - 1) * 16,
+            "synthesis + residual read, residual written")
+    stage_roofs["coupling_fp64"] = flop_entry(
+        "gm_coupling", "k_coupling (exact 0-0, 0-2, 2-2 3j rows, plain FP64 form)", 24.0 * _n3j(lmax), PEAK_FP64,
+        "fp64 valu", stages_fp64)
     roofline_fp64 = roof(stages_fp64, False)
     ana0_ms = stages_fp64.get("gm_lat_ana_s0", {}).get("mean_ms", float("nan"))
     cpl_ms = stages_fp64.get("gm_coupling", {}).get("mean_ms", float("nan"))
@@ -367,6 +447,7 @@ def main():
             from oracle.c import cpu_baseline
 
             cpu = cpu_baseline.estimate_pipeline(nside, lmax, target_seconds=args.cpu_seconds, log=log)
+            cpu["ladder"] = cpu_baseline.ladder((64, 256, 1024), log=log)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {exc!r}")
 
@@ -383,7 +464,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("f64" if args.form == "fp64" else
-                  "f64 recurrence, f32 contraction (f64 within 14.5 deg of the poles), f64 coupling and LU"),
+                  "f64 recurrence; f32 contraction in the first analysis and the syntheses, f16 hi/lo products in "
+                  "the refinement analyses (f64 within 14.5 deg of the poles); f64 coupling and LU"),
         "data": "synthetic",
         "form": args.form,
         "fp64_s": fp64_s,
@@ -393,8 +475,10 @@ def main():
         "config": {"workload": f"T+P full MASTER, HEALPix Nside {nside}, lmax {lmax}, n_iter 3, "
                                f"20 x 5deg holes C1 2deg, bins of {nlb}", "nside": nside, "lmax": lmax,
                    "n_iter": 3, "nlb": nlb, "parallelism": "single GPU"},
+        "alm_err_vs_fp64": alm_err,
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
+        "roofline_stages": stage_roofs,
         "stages": {
             "ana_s0_pair_fp64_frac": (f0 / (ana0_ms * 1e-3)) / PEAK_FP64,
             "coupling_fp64_ms": cpl_ms,

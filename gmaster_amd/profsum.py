@@ -16,7 +16,7 @@ import re
 import sys
 from collections import defaultdict
 
-_KERNELS = ("k_bat_ana", "k_bat_syn64", "k_bat_syn", "k_bat_finish", "k_bat_prep", "k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
+_KERNELS = ("k_bat_ana_h", "k_bat_ana", "k_bat_syn64", "k_bat_syn", "k_bat_finish", "k_bat_prep", "k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
             "k_coupling", "k_toeplitz", "k_ring_fwd_s0", "k_ring_fwd_s", "k_ring_inv_s0", "k_ring_inv_s", "k_fold_s0", "k_fold_s",
             "k_finish_s0", "k_finish_s", "k_prep_s0", "k_prep_s", "k_lat_tables", "k_cl_partial", "k_cl_final",
             "k_bin_block", "k_bin_cell", "k_gauss_legendre", "k_wigner_d_table", "k_wigner_d_corr",
@@ -59,6 +59,24 @@ def _per_dispatch(path, counter):
     return acc
 
 
+def _first_copy(path, counter):
+    """Counter value of the first runtime copy dispatch of at least 64 MiB in dispatch order:
+    bench.py's ``--calibrate-copy`` 1 GiB copy, which runs before any pipeline work (small
+    table uploads come earlier; later copies, such as clones of map inputs, can be larger)."""
+    best = None
+    for r in csv.DictReader(open(path)):
+        if r.get("Counter_Name", counter) != counter or "copybuffer" not in r["Kernel_Name"].lower():
+            continue
+        if float(r["Counter_Value"]) < 64 * 1024:
+            continue
+        d = int(r.get("Dispatch_Id", 0))
+        if best is None or d < best[0]:
+            best = (d, float(r["Counter_Value"]))
+    if best is None:
+        raise SystemExit("no copy kernel dispatch in the counter file")
+    return best[1]
+
+
 def pmc(path, counter):
     acc = defaultdict(list)
     for k, v in _per_dispatch(path, counter).items():
@@ -66,6 +84,10 @@ def pmc(path, counter):
     print(f"{'kernel':40s} {'dispatches':>10s} {'mean_KiB':>16s} {'max_KiB':>16s}   ({counter})")
     for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
         print(f"{k:40s} {len(v):10d} {sum(v) / len(v):16.1f} {max(v):16.1f}")
+    try:
+        print(f"# calibration copy (first copy dispatch): {_first_copy(path, counter):.1f} KiB")
+    except SystemExit:
+        pass
 
 
 def _is(name, kernel):
@@ -73,19 +95,12 @@ def _is(name, kernel):
     return name == kernel if "<" in kernel else name.split("<")[0] == kernel
 
 
-def _copy_max(acc):
-    vals = [max(v) for k, v in acc.items() if "copy" in k.lower()]
-    if not vals:
-        raise SystemExit("no copy kernel dispatch in the counter file")
-    return max(vals)
-
-
 def traffic(fetch_path, write_path, kernel, copy_bytes):
     copy_bytes = float(copy_bytes)
     out = {}
     for tag, path, counter in (("fetch", fetch_path, "FETCH_SIZE"), ("write", write_path, "WRITE_SIZE")):
         acc = _per_dispatch(path, counter)
-        ref_kib = _copy_max(acc)
+        ref_kib = _first_copy(path, counter)
         scale = copy_bytes / (ref_kib * 1024.0)
         vals = [x for k, v in acc.items() if _is(short(k), kernel) for x in v]
         if not vals:
@@ -109,12 +124,15 @@ def _summary_rows(path):
 
 
 def traffic_from_summaries(fetch_txt, write_txt, kernel, copy_bytes):
-    """As ``traffic``, from two ``pmc`` summaries (mean per kernel, max for the copy kernel)."""
+    """As ``traffic``, from two ``pmc`` summaries (mean per kernel; the calibration copy line)."""
     copy_bytes = float(copy_bytes)
     out = {}
     for tag, path, counter in (("fetch", fetch_txt, "FETCH_SIZE"), ("write", write_txt, "WRITE_SIZE")):
         rows = _summary_rows(path)
-        ref_kib = max(r[3] for r in rows if "copy" in r[0].lower() and "rocclr" in r[0].lower())
+        cal = [float(line.split(":", 1)[1].split()[0]) for line in open(path) if line.startswith("# calibration copy")]
+        if not cal:
+            raise SystemExit(f"{path} has no calibration copy line")
+        ref_kib = cal[0]
         hits = [r for r in rows if _is(short(r[0]), kernel)]
         if not hits:
             raise SystemExit(f"kernel {kernel} not found in {path}")

@@ -170,3 +170,35 @@ def estimate_pipeline(nside, lmax, target_seconds=20.0, log=print):
         "coupling_s": t_cpl,
         "ns_per_triple": t_triple * 1e9,
     }
+
+
+def ladder(nsides=(64, 256), log=print):
+    """Unsampled CPU timings at small Nside: one complete spin-2 Legendre analysis pass (every
+    order, every ring) and every row of the 3j coupling, scaled to the pipeline's 14 passes by
+    triple count as ``estimate_pipeline`` does.  Returns a list of dicts (seconds per pipeline)."""
+    cores = len(os.sched_getaffinity(0))
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    out = []
+    for nside in nsides:
+        lmax = 3 * nside - 1
+        L = lmax + 1
+        nring = 4 * nside - 1
+        rng = np.random.default_rng(nside)
+        W = 2 * L - 1
+        spec = (rng.standard_normal((nring, W)) + 1j * rng.standard_normal((nring, W))) * 1e-3
+        orders = np.arange(L, dtype=np.int32)
+        t = time.perf_counter()
+        lat_ana_s(nside, lmax, 2, spec, orders)
+        tl = time.perf_counter() - t
+        tri = L * (L + 1) / 2
+        t_leg = tl / (tri * nring) * (7 * tri * 2 * nside + 7 * tri * nring)
+        window = 1.0 / (1.0 + np.arange(L)) ** 2
+        t = time.perf_counter()
+        coupling_rows(lmax, window, orders)
+        tc = time.perf_counter() - t
+        out.append({"nside": nside, "lmax": lmax, "seconds": t_leg + tc, "legendre_pass_s": tl,
+                    "legendre_s": t_leg, "coupling_s": tc, "cores": cores})
+        log(f"cpu ladder: Nside {nside}: full spin-2 pass {tl:.2f}s, full coupling {tc:.2f}s "
+            f"-> {t_leg + tc:.1f}s per pipeline")
+    return out
+
