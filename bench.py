@@ -291,6 +291,7 @@ def main():
             run_pipeline(nmt, mask, tqu, nside, nlb, form)
             torch.cuda.synchronize()
             log(f"{form} warmup {i}: {time.time() - t:.3f}s")
+        torch.cuda.reset_peak_memory_stats(dev)
         timers = _lib.enable_timers(True) if with_timers else None
         times, info = [], []
         bp = None
@@ -306,10 +307,11 @@ def main():
             log(f"{form} step {i}: {times[-1]:.3f}s")
         if with_timers:
             _lib.enable_timers(False)
+        peak_mem[form] = torch.cuda.max_memory_allocated(dev) / 1e9
         return times, bp, timers, list(info)
 
     forms = ["fp64"] if args.form == "fp64" else ["fp64", args.form]
-    keep = {}
+    keep, peak_mem = {}, {}
     runs = {f: timed(f, True) for f in forms}
     times_fp64, bp_fp64, timers_fp64, _ = runs["fp64"]
     times, bp, timers, coupling_info = runs[args.form]
@@ -476,6 +478,7 @@ def main():
                                f"20 x 5deg holes C1 2deg, bins of {nlb}", "nside": nside, "lmax": lmax,
                    "n_iter": 3, "nlb": nlb, "parallelism": "single GPU"},
         "alm_err_vs_fp64": alm_err,
+        "peak_device_memory_gb": peak_mem,
         "roofline": roofline,
         "roofline_fp64": roofline_fp64,
         "roofline_stages": stage_roofs,
