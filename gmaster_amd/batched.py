@@ -185,10 +185,11 @@ def _like(t):
     return None if t is None else torch.empty_like(t)
 
 
-def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True):
+def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True, out=None):
     """``maps0`` ``(K0, npix)`` real spin-0 maps and ``maps2`` ``(K2, 2, npix)`` (Q, U) ->
     ``(alm0 (K0, nelem), alm2 (K2, 2, nelem))`` with ``n_iter`` Jacobi steps; either input
-    may be None."""
+    may be None.  ``out = (alm0, alm2)`` gives the output tensors (complex128, contiguous,
+    those shapes) instead of new ones."""
     dev = (maps0 if maps0 is not None else maps2).device
     maps0 = None if maps0 is None else maps0.to(device=dev, dtype=torch.float64).contiguous()
     maps2 = None if maps2 is None else maps2.to(device=dev, dtype=torch.float64).contiguous()
@@ -206,8 +207,13 @@ def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True):
     L = lmax + 1
     nelem = L * (L + 1) // 2
     spec0, spec2 = _fft_fwd(bp, maps0, maps2)
-    alm0 = torch.empty((k0, nelem), dtype=torch.complex128, device=dev) if k0 else None
-    alm2 = torch.empty((k2, 2, nelem), dtype=torch.complex128, device=dev) if k2 else None
+    if out is not None:
+        alm0, alm2 = out
+        _check_out(alm0, (k0, nelem), torch.complex128, dev)
+        _check_out(alm2, (k2, 2, nelem), torch.complex128, dev)
+    else:
+        alm0 = torch.empty((k0, nelem), dtype=torch.complex128, device=dev) if k0 else None
+        alm2 = torch.empty((k2, 2, nelem), dtype=torch.complex128, device=dev) if k2 else None
     analysis(bp, spec0, spec2, alm0, alm2, 0)
     if n_iter <= 0:
         return alm0, alm2
@@ -234,8 +240,18 @@ def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True):
     return alm0, alm2
 
 
-def alm2map(alm0, alm2, nside, lmax, polar_skip=True):
-    """Inverse of ``map2alm`` (no refinement): ``(maps0 (K0, npix), maps2 (K2, 2, npix))``."""
+def _check_out(t, shape, dtype, dev):
+    if shape[0] == 0:
+        if t is not None:
+            raise ValueError("output given for an absent kind")
+        return
+    if t is None or tuple(t.shape) != shape or t.dtype != dtype or t.device != dev or not t.is_contiguous():
+        raise ValueError(f"output must be a contiguous {dtype} tensor of shape {shape} on {dev}")
+
+
+def alm2map(alm0, alm2, nside, lmax, polar_skip=True, out=None):
+    """Inverse of ``map2alm`` (no refinement): ``(maps0 (K0, npix), maps2 (K2, 2, npix))``;
+    ``out = (maps0, maps2)`` gives the output tensors (float64, contiguous)."""
     dev = (alm0 if alm0 is not None else alm2).device
     k0 = 0 if alm0 is None else alm0.shape[0]
     k2 = 0 if alm2 is None else alm2.shape[0]
@@ -249,11 +265,14 @@ def alm2map(alm0, alm2, nside, lmax, polar_skip=True):
               spec0, spec2)
     st = _lib.stream_ptr(dev)
     maps0 = maps2 = None
+    if out is not None:
+        _check_out(out[0], (k0, npix), torch.float64, dev)
+        _check_out(out[1], (k2, 2, npix), torch.float64, dev)
     if k0:
-        maps0 = torch.empty((k0, npix), dtype=torch.float64, device=dev)
+        maps0 = out[0] if out is not None else torch.empty((k0, npix), dtype=torch.float64, device=dev)
         _lib.call("gm_ring_fft_inv_s0", _lib.ptr(spec0), k0, L, spec0.stride(0), nside, lmax, _lib.ptr(maps0), npix, st)
     if k2:
-        maps2 = torch.empty((k2, 2, npix), dtype=torch.float64, device=dev)
+        maps2 = out[1] if out is not None else torch.empty((k2, 2, npix), dtype=torch.float64, device=dev)
         for f in range(k2):
             _lib.call("gm_ring_fft_inv_s", _lib.ptr(spec2[f]), W, nside, lmax, _lib.ptr(maps2[f, 0]),
                       _lib.ptr(maps2[f, 1]), st)

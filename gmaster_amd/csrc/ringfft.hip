@@ -13,6 +13,10 @@
 // the fold kernel at the end of the file, with no FFT inside the refinement loop.
 #include <stdarg.h>
 
+#include <mutex>
+#include <utility>
+#include <vector>
+
 #include "common.h"
 
 namespace gm {
@@ -828,13 +832,24 @@ inline int threads_for(int N) {
 
 constexpr int kMaxNside = 8192;
 
+// Dynamic LDS above 64 KiB needs the kernel's limit raised.  It is raised once per kernel
+// and device, to the largest size any bucket uses, so that launches after the first make
+// no other runtime call (HIP graph capture records launches only).
 template <typename K>
 int set_lds(K kernel, size_t bytes) {
-    if (bytes > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess) return fail(GM_E_LAUNCH, "hipFuncSetAttribute: %s", hipGetErrorString(e));
-    }
+    if (bytes <= 64 * 1024) return GM_OK;
+    constexpr size_t kMax = (size_t)kLdsN * sizeof(double2);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(GM_E_LAUNCH, "hipGetDevice failed");
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, int>> done;
+    const void* fn = reinterpret_cast<const void*>(kernel);
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& d : done)
+        if (d.first == fn && d.second == dev) return GM_OK;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(bytes > kMax ? bytes : kMax));
+    if (e != hipSuccess) return fail(GM_E_LAUNCH, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    done.emplace_back(fn, dev);
     return GM_OK;
 }
 
