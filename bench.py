@@ -13,17 +13,22 @@ Workload (synthetic, seeded): mask = 1 except 20 random 5-degree disc holes, C1-
 sqrt(TT EE), C_BB = 0.01 C_EE; lmax = 3 nside - 1; bins of 16.
 
 Two forms are timed in the same run: plain FP64 and the chosen form (``--form``, default
-``fast``): Legendre passes with an FP64 recurrence and FP32 contraction, the Jacobi
-refinement on corrections, and the coupling from the Toeplitz approximation with a wide
-exact region checked by exact spot checks.  ``value`` is the chosen form's time; ``fp64_s``
-the plain form's, and ``bandpower_err_vs_fp64`` the chosen form's largest relative
-bandpower error per spectrum.
+``fast``): T, the mask and Q/U of the realisation through one shared FP64 recurrence with
+the contraction on the matrix unit (FP16 hi/lo products in the analyses, FP32 in the
+syntheses, FP64 near the poles), the Jacobi refinement on corrections, and the coupling
+from the Toeplitz approximation with a wide exact region checked by exact spot checks.
+``value`` is the chosen form's time; ``fp64_s`` the plain form's; ``bandpower_err_vs_fp64``
+and ``alm_err_vs_fp64`` the chosen form's largest relative errors.
 
 Prints one JSON line (value = seconds per pipeline, lower is better) with a roofline
-entry for the chosen form's dominant kernel (spin-2 Legendre analysis) measured with
-HIP events on the launching stream during the timed steps, and a CPU baseline (the
-project's C/OpenMP FP64 restatement of the same Legendre and 3j work, timed on a bounded
-sample of orders / rows at the same Nside and extrapolated to the whole pipeline).
+entry for the chosen form's dominant kernel (the batched analysis) and per-stage entries,
+measured with HIP events on the launching stream during the timed steps, and a CPU
+baseline (the project's C/OpenMP FP64 restatement of the same Legendre and 3j work, timed
+on a bounded sample of orders / rows at the same Nside and extrapolated to the whole
+pipeline, beside unsampled runs at Nside 64 / 256 / 1024).
+
+Other modes: ``--sim-batch N`` (config 3), ``--coupling-forms`` (config 4), ``--sims N``,
+``--sht-batch K``.
 """
 
 import argparse
@@ -232,6 +237,90 @@ def sims(args):
     }))
 
 
+def coupling_forms(args):
+    """Config 4: the mode coupling at the benchmark Nside by exact 3j, by the chosen form's
+    wide-band Toeplitz, by the committed surrogate's narrow Toeplitz alone and with its
+    learned residual.  Per mask (the benchmark mask and ``--fresh-masks`` random ones): the
+    time of the three workspaces' coupling, the relative Frobenius error of each block
+    against the exact one, and the bandpower errors of T+P fields decoupled with each
+    coupling against the exact decoupling of the same coupled spectra."""
+    import numpy as np
+    import torch
+
+    import gmaster_amd as nmt
+    from gmaster_amd import learned, workload, workspaces
+
+    nside, nlb = args.nside, args.nlb
+    lmax = 3 * nside - 1
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    sur = learned.committed_surrogate(dev)
+    narrow = sur.grid.toeplitz()
+    rng = np.random.default_rng(2024)
+    tqu = workload.tqu_maps(nside, lmax, seed=4321, device=dev)
+    b = nmt.NmtBin.from_nside_linear(nside, nlb)
+    masks = [("benchmark: 20 holes of 5 deg, C1 2 deg", workload.hole_mask(nside, nholes=20, radius_deg=5.0,
+                                                                          apo_deg=2.0, kind="C1", seed=1234,
+                                                                          device=dev))]
+    for i in range(args.fresh_masks):
+        masks.append((f"fresh random mask {i}", learned.random_mask(nside, rng, dev)))
+    variants = [("exact", "fp64", {}), ("toeplitz_wide", "fast", {}),
+                ("toeplitz_narrow", "fp64", {"l_toeplitz": narrow[0], "l_exact": narrow[1], "dl_band": narrow[2]}),
+                ("learned", "learned", {})]
+    out = []
+    for name, mask in masks:
+        nmt.set_form("fast")
+        fT = nmt.NmtField(mask, tqu[0:1], n_iter=3)
+        fP = nmt.NmtField(mask, tqu[1:3], n_iter=3)
+        cls = [nmt.compute_coupled_cell(f1, f2) for f1, f2 in ((fT, fT), (fT, fP), (fP, fP))]
+        rec = {"mask": name, "fsky": float(mask.mean())}
+        ref_bp = ref_blocks = None
+        for vname, form, kw in variants:
+            nmt.set_form(form)
+            workspaces.clear_coupling_cache()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ws = [nmt.NmtWorkspace.from_fields(f1, f2, b, **kw) for f1, f2 in ((fT, fT), (fT, fP), (fP, fP))]
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            bp = [w.decouple_cell(c) for w, c in zip(ws, cls)]
+            blocks = {k: ws[2].blocks[k] for k in ("00", "02", "22p", "22m")}
+            entry = {"coupling_s": dt, "info": ws[2].coupling_info}
+            if vname == "exact":
+                ref_bp, ref_blocks = bp, {k: v.clone() for k, v in blocks.items()}
+            else:
+                entry["block_frob_err"] = {k: float(torch.linalg.norm(blocks[k] - ref_blocks[k]) /
+                                                    torch.linalg.norm(ref_blocks[k])) for k in ref_blocks}
+                entry["bandpower_err"] = bandpower_errors(bp, ref_bp)
+                if vname == "learned":
+                    # the correction is zero above the residual grid (l > l_top): errors of the
+                    # learned 22 block by row range, against the narrow Toeplitz alone
+                    top = int(sur.grid.l_grid[-1])
+                    col = 2 * torch.arange(lmax + 1, dtype=torch.float64, device=dev) + 1
+                    for part, sl in (("l<=" + str(top), slice(0, top + 1)), ("l>" + str(top), slice(top + 1, None))):
+                        d = (blocks["22p"] - ref_blocks["22p"])[sl] / col
+                        r = ref_blocks["22p"][sl] / col
+                        entry.setdefault("rows", {})[part] = float(torch.linalg.norm(d) / torch.linalg.norm(r))
+            rec[vname] = entry
+            log(f"{name}: {vname}: coupling {dt:.3f}s " + (f"bandpower max err {entry['bandpower_err']['max']:.2e}"
+                                                           if vname != "exact" else ""))
+            del ws, bp
+        nmt.set_form("fp64")
+        workspaces.clear_coupling_cache()
+        out.append(rec)
+        del fT, fP, cls
+    bench_mask = out[0]
+    print(json.dumps({
+        "metric": f"Mode coupling at Nside {nside} (lmax {lmax}): exact 3j vs Toeplitz vs learned residual",
+        "value": bench_mask["learned"]["coupling_s"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "dtype": "f64 coupling, bf16-weight / fp32-accumulate network", "data": "synthetic",
+        "config": {"workload": f"00/02/22 workspaces of T+P fields at Nside {nside}, bins of {nlb}",
+                   "narrow_toeplitz": list(narrow), "wide_toeplitz": list(workspaces.nmt_params.fast_toeplitz),
+                   "surrogate_grid_l_top": int(sur.grid.l_grid[-1])},
+        "masks": out,
+    }))
+
+
 def sim_batch(args):
     """Config 3: N T+P realisations of mixed Nside (split evenly over ``--sim-nsides``),
     map2alm (n_iter 3) + alm2map of every map through the batched transforms, the whole job
@@ -325,13 +414,16 @@ def main():
     ap.add_argument("--nlb", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="target CPU sample time")
-    ap.add_argument("--form", default="fast", choices=["fp64", "fast"],
+    ap.add_argument("--form", default="fast", choices=["fp64", "fast", "learned"],
                     help="form timed for the metric (fp64 is always timed beside it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (scales PMC byte counters)")
     ap.add_argument("--sht-batch", type=int, default=0,
                     help="instead of the MASTER pipeline, time K spin-0 maps through the batched transforms")
     ap.add_argument("--max-group", type=int, default=2, help="maps per shared recurrence for --sht-batch")
+    ap.add_argument("--coupling-forms", action="store_true",
+                    help="config 4: exact vs Toeplitz vs learned coupling at --nside instead")
+    ap.add_argument("--fresh-masks", type=int, default=2, help="random masks beside the benchmark mask")
     ap.add_argument("--sim-batch", type=int, default=0,
                     help="config 3: time N T+P realisations of mixed Nside as one HIP graph instead")
     ap.add_argument("--sim-nsides", default="256,512,1024", help="Nside values of --sim-batch")
@@ -349,6 +441,8 @@ def main():
         return sims(args)
     if args.sim_batch > 0:
         return sim_batch(args)
+    if args.coupling_forms:
+        return coupling_forms(args)
 
     import torch
 
@@ -491,10 +585,10 @@ def main():
                 "frac": ach / peak, "flop_per_launch": flop, "mean_launch_ms": ms}
 
     def roof_residual(st):
-        # the refinement passes' analysis (the largest share of the step): per (l, m, ring
+        # the analyses on FP16 hi/lo products (the largest share of the step): per (l, m, ring
         # pair) 2 FP64 FMA of the recurrence + 12 channel FMA, the latter as 3 FP16 hi/lo
         # products on the matrix unit; op-mix peak of FP64 VALU and FP16 MFMA
-        e = flop_entry("gm_bat_ana_res", "k_bat_ana_h (residual T + mask + Q/U analysis, gm_bat_ana_res: FP16 "
+        e = flop_entry("gm_bat_ana_res", "k_bat_ana_h (T + mask + Q/U analysis, gm_bat_ana_res: FP16 "
                        "hi/lo ring blocks on the matrix unit + FP64 polar blocks)", 2 * (2 + 12) * T0,
                        28.0 / (4.0 / PEAK_FP64 + 3 * 24.0 / PEAK_F16), "fp64 valu + f16 mfma (3 hi/lo products)", st)
         e["pipe"] = ("FP64 VALU recurrence (78.6 TFLOP/s) + FP16 MFMA contraction (2.5 PFLOP/s, 3 products per "
@@ -511,6 +605,7 @@ def main():
     stage_roofs = {}
     if "gm_bat_ana" in stages:
         stage_roofs["first_analysis"] = roof_batched(stages)
+    if "gm_bat_syn" in stages:
         stage_roofs["synthesis"] = flop_entry(
             "gm_bat_syn", "k_bat_syn (T + mask + Q/U synthesis, gm_bat_syn: FP32 4x4x1 MFMA + FP64 polar pairs)",
             2 * (2 + 8) * T0, 20.0 / (4.0 / PEAK_FP64 + 16.0 / PEAK_FP32), "fp64 valu + fp32 mfma")
@@ -555,8 +650,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("f64" if args.form == "fp64" else
-                  "f64 recurrence; f32 contraction in the first analysis and the syntheses, f16 hi/lo products in "
-                  "the refinement analyses (f64 within 14.5 deg of the poles); f64 coupling and LU"),
+                  "f64 recurrence; f16 hi/lo matrix products in the analyses, f32 in the syntheses (f64 within "
+                  "14.5 deg of the poles); f64 coupling and LU"),
         "data": "synthetic",
         "form": args.form,
         "fp64_s": fp64_s,

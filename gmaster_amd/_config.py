@@ -41,7 +41,8 @@ def set_sht_calculator(calculator):
     nmt_params.sht_calculator = str(calculator)
 
 
-FORMS = ("fp64", "fast")
+FORMS = ("fp64", "fast", "learned")
+APPROXIMATE_FORMS = ("fast", "learned")  # forms with mixed-precision, batched transforms
 
 
 def set_form(form, toeplitz=None, tol=None):
@@ -53,6 +54,9 @@ def set_form(form, toeplitz=None, tol=None):
       checked by exact spot checks of random non-exact entries: when their largest error in
       correlation units exceeds ``tol`` (default 1e-3), the exact coupling is used instead.
       Below lmax = 2 l_toeplitz the exact coupling is used (it is cheap there).
+    * ``"learned"``: transforms as ``"fast"``; the coupling from the committed surrogate: the
+      Toeplitz approximation with its narrow exact region plus the network's prediction of
+      its residual, checked by the same exact spot checks (``tol``) with the exact fallback.
     """
     if form not in FORMS:
         raise ValueError(f"form must be one of {FORMS}")

@@ -17,7 +17,7 @@ import torch
 
 from . import _lib, sht
 from ._array import device_tensor, wrap
-from ._config import nmt_params
+from ._config import APPROXIMATE_FORMS, nmt_params
 
 
 def moore_penrose_pinvh(mat, tol_pinv):
@@ -85,7 +85,7 @@ def filter_alms(alms, spectra, lmax):
 
 
 def _precision():
-    return "mixed" if nmt_params.form == "fast" else "fp64"
+    return "mixed" if nmt_params.form in APPROXIMATE_FORMS else "fp64"
 
 
 def _map2alm(maps, spin, lmax, n_iter):

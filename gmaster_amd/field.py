@@ -16,7 +16,7 @@ from . import _lib, batched, sht
 from ._array import device_tensor, wrap
 from .deproject import deproject
 from .purify import purify
-from ._config import nmt_params
+from ._config import APPROXIMATE_FORMS, nmt_params
 from .utils import NmtAlmInfo, NmtMapInfo
 
 # Mask alms shared by fields built on masks with the same content (a 128-bit fingerprint of
@@ -44,7 +44,7 @@ def _mask_key(mask, lmax_mask, n_iter_mask, precision=None):
 def _precision():
     """Transform precision of the selected form: ``"fast"`` contracts in FP32 over an FP64
     recurrence (``sht._refine``), ``"fp64"`` keeps every pass in FP64."""
-    return "mixed" if nmt_params.form == "fast" else "fp64"
+    return "mixed" if nmt_params.form in APPROXIMATE_FORMS else "fp64"
 
 
 def clear_mask_cache():
@@ -66,7 +66,7 @@ _PENDING = []
 
 
 def _batchable(nside, lmax):
-    return nmt_params.form == "fast" and nmt_params.batch_fields and lmax >= 2 and nside <= 4096
+    return nmt_params.form in APPROXIMATE_FORMS and nmt_params.batch_fields and lmax >= 2 and nside <= 4096
 
 
 def flush_pending():

@@ -330,6 +330,46 @@ def default_path():
     return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "coupling_surrogate.safetensors")
 
 
+_SURROGATE = {}
+_LEARNED_CACHE = {}
+
+
+def committed_surrogate(device="cuda"):
+    """The committed network (``default_path()``), loaded once per device."""
+    dev = torch.device(device)
+    key = str(dev)
+    if key not in _SURROGATE:
+        _SURROGATE[key] = Surrogate.load(default_path(), device=dev)
+    return _SURROGATE[key]
+
+
+def learned_coupling_blocks(window, lmax, tol=None, n_check=4096, seed=0):
+    """The ``"learned"`` form's coupling blocks: the committed surrogate's Toeplitz
+    approximation plus its predicted residual, spot-checked against exact 3j entries with
+    the exact fallback above ``tol`` (default: the fast form's).  Cached per (lmax, window)
+    so that the 00 / 02 / 22 workspaces of one mask share one pass."""
+    import hashlib
+
+    from ._config import nmt_params
+
+    w = device_tensor(window)
+    key = (int(lmax), str(w.device), hashlib.sha1(w.cpu().numpy().tobytes()).hexdigest())
+    if key in _LEARNED_CACHE:
+        return _LEARNED_CACHE[key]
+    sur = committed_surrogate(w.device)
+    blocks, info = sur.approximate(w, lmax, tol=nmt_params.fast_tol if tol is None else tol, n_check=n_check,
+                                   seed=seed)
+    info = dict(info, form="learned", toeplitz=sur.grid.toeplitz(), tol=nmt_params.fast_tol if tol is None else tol)
+    if len(_LEARNED_CACHE) >= 2:
+        _LEARNED_CACHE.clear()
+    _LEARNED_CACHE[key] = (blocks, info)
+    return blocks, info
+
+
+def clear_learned_cache():
+    _LEARNED_CACHE.clear()
+
+
 def main(argv=None):
     """Train the surrogate on random masks: ``python -m gmaster_amd.learned --masks 4096 --out PATH``."""
     import argparse

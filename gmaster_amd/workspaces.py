@@ -150,6 +150,9 @@ def fast_coupling_blocks(window, lmax, toeplitz=None, tol=None, n_check=4096, se
 
 def clear_coupling_cache():
     _COUPLING_CACHE.clear()
+    from .learned import clear_learned_cache
+
+    clear_learned_cache()
 
 
 _COUPLING_PRECISION = ["auto"]
@@ -495,6 +498,9 @@ class NmtWorkspace:
             self.coupling_info = {"form": "exact+pure"}
         elif self.toeplitz is None and nmt_params.form == "fast":
             self.blocks, self.coupling_info = fast_coupling_blocks(self.pcl_mask, self.lmax)
+        elif self.toeplitz is None and nmt_params.form == "learned":
+            from .learned import learned_coupling_blocks
+            self.blocks, self.coupling_info = learned_coupling_blocks(self.pcl_mask, self.lmax)
         else:
             self.blocks = coupling_blocks(self.pcl_mask, self.lmax, self.toeplitz)
         self.layout = _layout(self.ncls, self.spin1, self.spin2, self.purity)

@@ -92,3 +92,45 @@ def test_committed_surrogate_beats_toeplitz_on_fresh_masks():
         print(f"nside {nside}: Frobenius error Toeplitz {et:.3e} learned {el:.3e}", flush=True)
         gains.append(et / el)
     assert min(gains) > 1.0 and float(np.median(gains)) > 2.0
+
+
+def test_learned_form_workspaces_and_fallback():
+    """``set_form("learned")``: workspaces take the committed surrogate's blocks (narrow
+    Toeplitz + learned residual) when the exact spot check passes, and the exact coupling
+    when it does not (a tolerance of 1e-12 forces the fallback); Nside 256, one random mask."""
+    import gmaster_amd as nmt
+    from gmaster_amd import learned, workload
+    from gmaster_amd.workspaces import clear_coupling_cache, coupling_blocks
+
+    nside = 256
+    lmax = 3 * nside - 1
+    rng = np.random.default_rng(31)
+    mask = learned.random_mask(nside, rng)
+    tqu = workload.tqu_maps(nside, lmax, seed=5)
+    b = nmt.NmtBin.from_nside_linear(nside, 16)
+    try:
+        nmt.set_form("learned")
+        f = nmt.NmtField(mask, tqu[1:3], n_iter=3)
+        clear_coupling_cache()
+        w = nmt.NmtWorkspace.from_fields(f, f, b)
+        info = w.coupling_info
+        print(f"learned form info {info}", flush=True)
+        assert info["form"] == "learned"
+        win = w.pcl_mask
+        clear_coupling_cache()
+        exact = {k: v.clone() for k, v in coupling_blocks(win, lmax).items()}
+        if not info["fallback"]:
+            sur = learned.committed_surrogate()
+            clear_coupling_cache()
+            toe = {k: v.clone() for k, v in coupling_blocks(win, lmax, sur.grid.toeplitz()).items()}
+            el, et = _frob_err(w.blocks, exact, lmax), _frob_err(toe, exact, lmax)
+            print(f"  Frobenius error: learned {el:.3e}, Toeplitz {et:.3e}", flush=True)
+            assert el < et
+        clear_coupling_cache()
+        blocks, info2 = learned.learned_coupling_blocks(win, lmax, tol=1e-12)
+        assert info2["fallback"]
+        for k in exact:
+            assert torch.equal(blocks[k], exact[k]), k
+    finally:
+        nmt.set_form("fp64")
+        clear_coupling_cache()

@@ -2,31 +2,23 @@
 (``gmaster_amd.simbatch``): the graph equals the eager run on every replay, with eager work
 and synchronisations between replays, and after new maps are drawn into the job's buffers;
 every realisation's alms (map2alm, n_iter 3) and output maps (alm2map) agree with the
-per-spin FP64 transforms of the same maps within the batched form's 2e-6 of the largest
-alm of each order (band of 16 orders; 5e-6 for spin-2 maps, relative to the largest
-value); padding maps stay zero; the batch-size tuner picks supported sizes."""
+per-spin FP64 transforms of the same maps within 1e-6 of the largest alm (map value) of
+that realisation and component; padding maps stay zero; the batch-size tuner picks
+supported sizes.  The realisations have red spectra (C_l ~ (l + 10)^-2), whose high
+orders hold alms ~1/l of the low ones, so the error scale is the largest alm of the map
+(as the bench's alm errors), not each order's own largest alm."""
 
-import numpy as np
 import pytest
 import torch
 
 pytestmark = pytest.mark.gpu
 
-TOL = 2e-6
-TOL_MAP2 = 5e-6
+TOL = 1e-6
 KEYS = ("alm_t", "alm_p", "out_t", "out_p")
 
 
-def _per_order_err(got, ref, lmax, band=16):
-    got, ref = got.cpu().numpy(), ref.cpu().numpy()
-    L = lmax + 1
-    start = np.concatenate([[0], np.cumsum(L - np.arange(L))])
-    scale = np.array([np.max(np.abs(ref[start[m]:start[m + 1]])) for m in range(L)])
-    worst = 0.0
-    for m in range(L):
-        sc = max(np.max(scale[max(0, m - band):m + 1]), 1e-300)
-        worst = max(worst, float(np.max(np.abs(got[start[m]:start[m + 1]] - ref[start[m]:start[m + 1]])) / sc))
-    return worst
+def _rel(got, ref):
+    return float((got - ref).abs().max() / ref.abs().max())
 
 
 def _snap(job):
@@ -85,18 +77,18 @@ def test_simjob_graph_replays_and_fp64_parity():
         lmax, nside, n = p["lmax"], p["nside"], p["n"]
         rt = sht.map2alm_s0(p["maps_t"][:n], lmax, n_iter=3, precision="fp64")
         for i in range(n):
-            worst["alm_t"] = max(worst["alm_t"], _per_order_err(p["alm_t"][i], rt[i], lmax))
+            worst["alm_t"] = max(worst["alm_t"], _rel(p["alm_t"][i], rt[i]))
             rp = sht.map2alm_spin(p["maps_p"][i], lmax, 2, n_iter=3, precision="fp64")
             for c in range(2):
-                worst["alm_p"] = max(worst["alm_p"], _per_order_err(p["alm_p"][i, c], rp[c], lmax))
+                worst["alm_p"] = max(worst["alm_p"], _rel(p["alm_p"][i, c], rp[c]))
             mt = sht.alm2map_s0(p["alm_t"][i:i + 1], nside, lmax)[0]
-            worst["out_t"] = max(worst["out_t"], float((p["out_t"][i] - mt).abs().max() / mt.abs().max()))
+            worst["out_t"] = max(worst["out_t"], _rel(p["out_t"][i], mt))
             mp = sht.alm2map_spin(p["alm_p"][i], nside, lmax, 2)
-            worst["out_p"] = max(worst["out_p"], float((p["out_p"][i] - mp).abs().max() / mp.abs().max()))
+            worst["out_p"] = max(worst["out_p"], _rel(p["out_p"][i], mp))
         if p["maps_t"].shape[0] > n:
             assert torch.count_nonzero(p["maps_t"][n:]) == 0 and torch.count_nonzero(p["alm_t"][n:]) == 0
     print(f"  worst vs FP64: {worst}", flush=True)
-    assert worst["alm_t"] < TOL and worst["alm_p"] < TOL and worst["out_t"] < TOL and worst["out_p"] < TOL_MAP2
+    assert max(worst.values()) < TOL, worst
 
 
 def test_tuner_picks_supported_sizes():
