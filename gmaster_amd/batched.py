@@ -231,6 +231,9 @@ def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True, out=None):
     d0, d2 = _like(alm0), _like(alm2)
     sign = 1
     for it in range(int(n_iter)):
+        if it == n_iter - 1:  # the last correction goes straight into the alms
+            analysis(bp, ra0, ra2, alm0, alm2, 1 if sign > 0 else 2, residual=True)
+            break
         analysis(bp, ra0, ra2, d0, d2, 0, residual=True)
         for a, d in ((alm0, d0), (alm2, d2)):
             if a is not None:
@@ -238,11 +241,10 @@ def map2alm(maps0, maps2, lmax, n_iter=0, polar_skip=True, out=None):
                     a.sub_(d)
                 else:
                     a.add_(d)
-        if it < n_iter - 1:
-            synthesis(bp, d0, d2, syn0, syn2)
-            _fold(bp, syn0, syn2, ra0, ra2, rb0, rb2)
-            ra0, rb0, ra2, rb2 = rb0, ra0, rb2, ra2
-            sign = -sign
+        synthesis(bp, d0, d2, syn0, syn2)
+        _fold(bp, syn0, syn2, ra0, ra2, rb0, rb2)
+        ra0, rb0, ra2, rb2 = rb0, ra0, rb2, ra2
+        sign = -sign
     return alm0, alm2
 
 

@@ -106,37 +106,69 @@ __global__ void k_coupling_pairs(const double* __restrict__ Wp, int lmax_w, cons
 //   le <= j <= lt, i >= n - 1 - lt + j -> c_e[le + i - j] sqrt(D_i D_j),  c_e[k] = S[k][le] / sqrt(D_k D_le)
 // for i = max(row, col), j = min(row, col); band and diagonal read S from the upper
 // triangle exactly as the reference's diag(S, offset) does.
-__global__ void k_toeplitz(const double* __restrict__ M, int n, int lt, int le, int dl, double* __restrict__ out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
-    if (b >= n) return;
-    const int i = a > b ? a : b, j = a > b ? b : a;
+//
+// A workgroup fills a tile of kTR rows x kTC columns.  D_k, c_t and c_e depend on one index
+// each and come from the diagonal and two columns of M (strided reads), so the tile stages
+// the D of its rows and columns and c_t, c_e over its range of i - j in LDS once, instead
+// of every element reading them from their own cache lines.
+constexpr int kTR = 32, kTC = 256;
+
+__global__ __launch_bounds__(kTC) void k_toeplitz(const double* __restrict__ M, int n, int lt, int le, int dl,
+                                                 double* __restrict__ out) {
+    __shared__ double Dc[kTC], Dr[kTR], ctv[kTC + kTR], cev[kTC + kTR];
+    const int b0 = blockIdx.x * kTC, a0 = blockIdx.y * kTR;
+    const int tid = threadIdx.x;
     auto S = [&](int r, int c) { return M[(int64_t)r * n + c] / (2.0 * c + 1.0); };
     auto D = [&](int k) { return fabs(S(k, k)); };
-    const double Di = D(i), Dj = D(j);
-    double v;
+    const int amax = min(a0 + kTR, n) - 1, bmax = min(b0 + kTC, n) - 1;
+    int dlo, dhi;  // range of |row - col| over the tile
+    if (bmax < a0) {
+        dlo = a0 - bmax;
+        dhi = amax - b0;
+    } else if (b0 > amax) {
+        dlo = b0 - amax;
+        dhi = bmax - a0;
+    } else {
+        dlo = 0;
+        dhi = max(amax - b0, bmax - a0);
+    }
     {
-        const int k = (i - j + lt) % n;
-        const double den = sqrt(D(k) * D(lt));
-        v = (den > 0.0 ? S(k, lt) / den : 0.0) * sqrt(Di * Dj);
+        const int b = b0 + tid;
+        Dc[tid] = b < n ? D(b) : 0.0;
+        if (tid < kTR) Dr[tid] = a0 + tid < n ? D(a0 + tid) : 0.0;
+        const double Dlt = D(lt), Dle = D(le);
+        for (int t = tid; t <= dhi - dlo; t += kTC) {
+            const int dd = dlo + t;
+            const int kt = (dd + lt) % n;
+            const double dent = sqrt(D(kt) * Dlt);
+            ctv[t] = dent > 0.0 ? S(kt, lt) / dent : 0.0;
+            const int ke = le + dd;
+            double ce = 0.0;
+            if (ke < n) {
+                const double Dk = D(ke);
+                const double den = sqrt(Dk * Dle);
+                ce = Dk > 0.0 ? S(ke, le) / den : 0.0;  // guarded by D_k only, as the reference
+            }
+            cev[t] = ce;
+        }
     }
-    if (j <= le) v = S(i, j);
-    if (i - j <= dl) v = (i == j) ? S(i, i) : S(j, i);
-    if (i >= lt && j >= lt) {
-        const int k = (i - j + lt) % n;
-        const double den = sqrt(D(k) * D(lt));
-        v = (den > 0.0 ? S(k, lt) / den : 0.0) * sqrt(Di * Dj);
+    __syncthreads();
+    const int b = b0 + tid;
+    if (b >= n) return;
+    for (int r = 0; r < kTR && a0 + r < n; ++r) {
+        const int a = a0 + r;
+        const int i = a > b ? a : b, j = a > b ? b : a;
+        const double Di = a > b ? Dr[r] : Dc[tid], Dj = a > b ? Dc[tid] : Dr[r];
+        const int t = i - j - dlo;
+        double v = ctv[t] * sqrt(Di * Dj);
+        if (j <= le) v = S(i, j);
+        if (i - j <= dl) v = (i == j) ? S(i, i) : S(j, i);
+        if (i >= lt && j >= lt) v = ctv[t] * sqrt(Di * Dj);
+        if (i == j) v = S(i, i);
+        if (i - j >= 1 && i - j <= dl && j <= lt) v = S(j, i);
+        if (j >= le && j <= lt && i >= n - 1 - lt + j) v = cev[t] * sqrt(Dj * Di);
+        out[(int64_t)a * n + b] = v * (2.0 * b + 1.0);
     }
-    if (i == j) v = S(i, i);
-    if (i - j >= 1 && i - j <= dl && j <= lt) v = S(j, i);
-    if (j >= le && j <= lt && i >= n - 1 - lt + j) {
-        const int k = le + i - j;
-        const double Dk = D(k);
-        const double den = sqrt(Dk * D(le));
-        const double ce = Dk > 0.0 ? S(k, le) / den : 0.0;  // guarded by D_k only, as the reference
-        v = ce * sqrt(Dj * Di);
-    }
-    out[(int64_t)a * n + b] = v * (2.0 * b + 1.0);
 }
 
 // Which fill of the Toeplitz approximation sets element (i >= j): 0 = an exact value,
@@ -223,7 +255,8 @@ extern "C" int gm_toeplitz(const double* M, int64_t lmax, int64_t l_toeplitz, in
         l_toeplitz >= lmax || dl_band >= lmax)
         return fail(GM_E_ARG, "gm_toeplitz: need 0 < l_exact <= l_toeplitz < lmax and 0 <= dl_band < lmax");
     const int n = (int)lmax + 1;
-    hipLaunchKernelGGL(k_toeplitz, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_toeplitz, dim3((unsigned)((n + kTC - 1) / kTC), (unsigned)((n + kTR - 1) / kTR)), dim3(kTC), 0,
+                       (hipStream_t)stream,
                        M, n, (int)l_toeplitz, (int)l_exact, (int)dl_band, out);
     return check_launch("gm_toeplitz");
 }

@@ -11,9 +11,10 @@ Restates ``gmaster/workspaces.py``:
 * the slot / sign layout of the blocks (``:1851-1903``), binning with weights, f_ell and
   beams (``:1452-1470, :292-310``) -- kernel ``gm_bin_block``;
 * ``decouple_cell`` (``:2106-2147``): binned spectra (``gm_bin_cell``) and an FP64 LU
-  solve of the binned MCM: its LU factors (``torch.linalg.lu_factor``, getrf on the GPU)
-  are computed once per binned matrix and reused by every ``decouple_cell`` and by the
-  bandpower windows (getrs).
+  solve of the binned MCM: its LU factors (``gm_lu_factor``: blocked, without row
+  exchanges, with a flag that falls back to ``torch.linalg.lu_factor`` when partial
+  pivoting would exchange rows) are computed once per binned matrix and reused by every
+  ``decouple_cell`` (``gm_lu_solve``) and by the bandpower windows (getrs).
 """
 
 import hashlib
@@ -314,6 +315,37 @@ def _layout(ncls, spin1, spin2, purity=(False, False, False, False)):
     return out
 
 
+
+SOLVE_RHS_MAX = 64  # right-hand sides solved by gm_lu_solve; more go through getrs
+
+
+def lu_factor(M):
+    """LU factors of a square FP64 device matrix for ``lu_solve``: ``gm_lu_factor`` (no row
+    exchanges, the binned coupling matrices being near the identity) unless its flag says
+    that partial pivoting would exchange rows, then ``torch.linalg.lu_factor``."""
+    n = M.shape[0]
+    lu = M.contiguous().clone()
+    flag = torch.zeros(1, dtype=torch.int32, device=M.device)
+    _lib.call("gm_lu_factor", _lib.ptr(lu), n, _lib.ptr(flag), _lib.stream_ptr(M.device))
+    if int(flag.item()) != 0:
+        return ("pivoted", torch.linalg.lu_factor(M))
+    return ("plain", lu)
+
+
+def lu_solve(fac, rhs):
+    """``M^-1 rhs`` for ``fac = lu_factor(M)`` and ``rhs`` ``(n, k)``."""
+    kind, f = fac
+    if kind == "pivoted":
+        return torch.linalg.lu_solve(*f, rhs)
+    n = f.shape[0]
+    if rhs.shape[1] > SOLVE_RHS_MAX:
+        piv = torch.arange(1, n + 1, dtype=torch.int32, device=f.device)
+        return torch.linalg.lu_solve(f, piv, rhs)
+    out = rhs.to(torch.float64).contiguous().clone()
+    _lib.call("gm_lu_solve", _lib.ptr(f), n, _lib.ptr(out), out.shape[1], _lib.stream_ptr(f.device))
+    return out
+
+
 class NmtWorkspace:
     """Mode-coupling matrix, its binned form and the bandpower operators of two fields."""
 
@@ -551,8 +583,8 @@ class NmtWorkspace:
     def _solve(self, rhs):
         """``mcm_binned^-1 rhs`` from LU factors computed once per binned matrix."""
         if getattr(self, "_lu", None) is None:
-            self._lu = torch.linalg.lu_factor(self.mcm_binned)
-        return torch.linalg.lu_solve(*self._lu, rhs)
+            self._lu = lu_factor(self.mcm_binned)
+        return lu_solve(self._lu, rhs)
 
     def couple_cell(self, cl_in):
         """Theory ``(ncls, >= lmax+1)`` -> expected coupled spectra ``(ncls, lmax+1)``."""

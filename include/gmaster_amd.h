@@ -240,6 +240,17 @@ int gm_coupling_pairs(const double* window, int64_t lmax_window, const double* g
 int gm_toeplitz(const double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exact, int64_t dl_band,
                 double* out, void* stream);
 
+/* In-place LU of the row-major n x n matrix A (unit lower L below the diagonal, U on and
+ * above) without row exchanges; *flag (device int32) is cleared and then set nonzero when a
+ * pivot is zero or some |L_ij| > 1, i.e. when partial pivoting would have exchanged rows
+ * (the factors are then unusable and a pivoting LU is needed).  Replaces the factorisation
+ * inside jnp.linalg.solve(mcm_binned, ...) of gmaster/workspaces.py:2146 (decouple_cell). */
+int gm_lu_factor(double* A, int64_t n, int32_t* flag, void* stream);
+
+/* B <- U^-1 L^-1 B for the factors of gm_lu_factor and row-major B (n x nrhs), in place.
+ * Replaces the triangular solves of the same jnp.linalg.solve. */
+int gm_lu_solve(const double* LU, int64_t n, double* B, int64_t nrhs, void* stream);
+
 /* Binned block: out[b1][b2] = sum_{l1 in b1, l2 in b2} wout[l1] M[l1][l2] tin[l2]
  * (CSR bands: ells sorted by band, band_start has nbands+1 entries).
  * Replaces gmaster/workspaces.py:292-310 (_BlockMCM.banded). */

@@ -194,6 +194,27 @@ def test_toeplitz_kernel_matches_oracle(lt, le, dl):
         assert err < 1e-14
 
 
+@pytest.mark.parametrize("lmax,lt,le,dl", [(600, 300, 40, 20), (777, 500, 120, 64), (1000, 513, 513, 0)])
+def test_toeplitz_kernel_tiles_match_oracle(lmax, lt, le, dl):
+    """Blocks wider than one 32 x 256 tile of the fill kernel (tiles straddling the diagonal,
+    the band, the exact columns and the corner extrapolation, ragged last tiles), on a
+    random matrix with a positive diagonal, against the oracle's fill."""
+    from gmaster_amd import _lib
+    from oracle.toeplitz import apply_to_mcm_block
+
+    rng = np.random.default_rng(lmax)
+    M = rng.uniform(-1.0, 1.0, size=(lmax + 1, lmax + 1)) * 1e-3
+    M[np.diag_indices(lmax + 1)] = rng.uniform(0.5, 2.0, size=lmax + 1)
+    Md = torch.as_tensor(M, device="cuda")
+    out = torch.empty_like(Md)
+    print(f"toeplitz tiles lmax={lmax} lt={lt} le={le} dl={dl}", flush=True)
+    _lib.call("gm_toeplitz", _lib.ptr(Md), lmax, lt, le, dl, _lib.ptr(out), _lib.stream_ptr("cuda"))
+    ref = apply_to_mcm_block(M, lt, le, dl)
+    err = _rel(out.cpu().numpy(), ref)
+    print(f"  rel={err:.2e}", flush=True)
+    assert err < 1e-14
+
+
 @pytest.mark.parametrize("lt,le,dl", TOEPLITZ[:2])
 def test_master_coefficients_with_toeplitz_match_oracle(lt, le, dl):
     """The reference's own check (tests/test_workspaces.py:89-105) at lmax 12, held against
