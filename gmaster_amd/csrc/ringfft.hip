@@ -180,6 +180,178 @@ __device__ __forceinline__ void dft_pass(double2* buf, int N, int p, int Ns, dou
     __syncthreads();
 }
 
+// ------------------------------------------------------------------ Bluestein pass
+// A prime radix p >= kBluesteinMin as the last pass (rem == p, Ns = J = N / p) runs as a
+// chirp-z convolution (Bluestein 1970) instead of Horner's O(N p): with
+// c_k = e^{sign i pi k^2 / p}, the DFT X_q = sum_u y_u e^{sign 2 pi i q u / p} equals
+// c_q sum_u (y_u c_u) conj(c_{q-u}), a circular convolution of length M >= 2p - 1 (a power
+// of two) done by radix-4/2 FFTs in LDS.  y_u = x[j + u J] e^{sign 2 pi i j u / N} for
+// residue j, and its outputs return to positions j + q J, as in the Horner pass.  The
+// ring's N points are parked in registers (T >= N / 8: at most 8 per thread) so that LDS
+// holds the M-point buffer; the filter spectrum FFT(conj c) is computed once per ring and
+// kept in registers (M / T <= 8 per thread).
+#ifndef GM_BLUESTEIN_MIN
+#define GM_BLUESTEIN_MIN 1024
+#endif
+constexpr int kBluesteinMin = GM_BLUESTEIN_MIN;
+
+// e^{sign i pi k^2 / p}, the phase reduced mod 2 p in integers
+__device__ inline double2 chirp(int k, int p, double sign) {
+    const int r = (k * k) % (2 * p);  // k < 8192
+    double sn, cs;
+    sincospi(sign * (double)r / (double)p, &sn, &cs);
+    return make_double2(cs, sn);
+}
+
+// e^{sign 2 pi i k / M} for 0 <= k < M / 2 from tw[k] = e^{-2 pi i k / M}, k <= M / 8
+// (the other octants by e^{-2 pi i (M/4 - k) / M} = i conj(...) and e^{-i pi / 2} = -i).
+__device__ inline double2 tw_get(const double2* tw, int k, int M, double sign) {
+    const int q = M >> 2, e = M >> 3;
+    const bool upper = k >= q;  // times -i
+    if (upper) k -= q;
+    double2 w = k <= e ? tw[k] : make_double2(-tw[q - k].y, -tw[q - k].x);  // -i conj(tw[q - k])
+    if (upper) w = make_double2(w.y, -w.x);
+    if (sign > 0.0) w.y = -w.y;
+    return w;
+}
+
+// One Stockham pass of radix P in {2, 4} over a power-of-two length M, twiddles from tw.
+template <int P>
+__device__ void radix_pass_tab(double2* buf, int M, int Ns, double sign, const double2* tw) {
+    const int J = M / P;
+    const int T = blockDim.x;
+    constexpr int MB = 8 / P;
+    const int step = M / (Ns * P);
+    double2 v[MB][P];
+#pragma unroll
+    for (int t = 0; t < MB; ++t) {
+        const int j = threadIdx.x + t * T;
+        if (j < J) {
+            const int k = j % Ns;
+#pragma unroll
+            for (int q = 0; q < P; ++q) v[t][q] = buf[j + q * J];
+            if (Ns > 1) {
+                const double2 w1 = tw_get(tw, k * step, M, sign);
+                double2 w = w1;
+#pragma unroll
+                for (int q = 1; q < P; ++q) {
+                    v[t][q] = cmul(v[t][q], w);
+                    w = cmul(w, w1);
+                }
+            }
+            dft_small<P>(v[t], sign);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MB; ++t) {
+        const int j = threadIdx.x + t * T;
+        if (j < J) {
+            const int k = j % Ns;
+            const int base = (j / Ns) * Ns * P + k;
+#pragma unroll
+            for (int q = 0; q < P; ++q) buf[base + q * Ns] = v[t][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Power-of-two FFT of length M in LDS (radix 4 passes, one radix 2 when log2 M is odd).
+__device__ __forceinline__ void fft_pow2(double2* buf, int M, double sign, const double2* tw) {
+    int Ns = 1, rem = M;
+    while (rem > 1) {
+        if (rem % 4 == 0) {
+            radix_pass_tab<4>(buf, M, Ns, sign, tw);
+            Ns *= 4;
+            rem /= 4;
+        } else {
+            radix_pass_tab<2>(buf, M, Ns, sign, tw);
+            Ns *= 2;
+            rem /= 2;
+        }
+    }
+}
+
+__device__ inline int bluestein_len(int p) {
+    int M = 1;
+    while (M < 2 * p - 1) M <<= 1;
+    return M;
+}
+
+// Whether the last pass of radix p over N points runs as a convolution: p large enough,
+// at least two residues (so M <= the next power of two of N, which the launch provides
+// in LDS) and threads for the M-point FFT and the parked data.
+__device__ inline bool use_bluestein(int N, int p) {
+    if (p < kBluesteinMin || N / p < 2) return false;
+    const int M = bluestein_len(p);
+    const int T = blockDim.x;
+    return M <= 8 * T && N <= 8 * T;
+}
+
+// tw: M / 8 + 1 points of LDS beside the M-point buffer for the twiddle table.
+__device__ __forceinline__ void bluestein_pass(double2* buf, int N, int p, double sign, double2* tw) {
+    const int T = blockDim.x, tid = threadIdx.x;
+    const int J = N / p;
+    const int M = bluestein_len(p);
+    for (int k = tid; k <= M / 8; k += T) tw[k] = twiddle(k, M, -1.0);
+    double2 park[8], bf[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = tid + k * T;
+        park[k] = i < N ? buf[i] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    // filter conj(c_k) on k in (-p, p), wrapped into M points, and its spectrum
+    for (int k = tid; k < M; k += T) {
+        const int kk = k < p ? k : (k > M - p ? M - k : -1);
+        buf[k] = kk >= 0 ? chirp(kk, p, -sign) : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    fft_pow2(buf, M, -1.0, tw);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = tid + k * T;
+        bf[k] = i < M ? buf[i] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const double inv_m = 1.0 / (double)M;
+    for (int j = 0; j < J; ++j) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = tid + k * T;
+            if (i < N && i % J == j) {
+                const int u = i / J;
+                buf[u] = cmul(cmul(park[k], twiddle((int64_t)j * u, N, sign)), chirp(u, p, sign));
+            }
+        }
+        for (int k = p + tid; k < M; k += T) buf[k] = make_double2(0.0, 0.0);
+        __syncthreads();
+        fft_pow2(buf, M, -1.0, tw);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = tid + k * T;
+            if (i < M) buf[i] = cmul(buf[i], bf[k]);
+        }
+        __syncthreads();
+        fft_pow2(buf, M, 1.0, tw);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int i = tid + k * T;
+            if (i < N && i % J == j) {
+                const int q = i / J;
+                park[k] = cscale(cmul(buf[q], chirp(q, p, sign)), inv_m);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int i = tid + k * T;
+        if (i < N) buf[i] = park[k];
+    }
+    __syncthreads();
+}
+
 __device__ int smallest_factor(int n) {
     if (n % 2 == 0) return 2;
     for (int f = 3; f * f <= n; f += 2)
@@ -193,7 +365,10 @@ __device__ int smallest_factor(int n) {
 // the FMA Horner step above, it returned spectra off by up to ~1e-7 relative on gfx950
 // (rings that never reach the direct pass included), while the inlined form matches
 // numpy to ~1e-16; the cause was not isolated.
-__device__ __forceinline__ void fft_lds(double2* buf, int N, double sign) {
+// BS: prime passes >= kBluesteinMin may run as a Bluestein convolution (tw: its twiddle
+// table in LDS, M / 8 + 1 points past the data buffer).
+template <bool BS = false>
+__device__ __forceinline__ void fft_lds(double2* buf, int N, double sign, double2* tw = nullptr) {
     int Ns = 1, rem = N;
     while (rem > 1) {
         int p;
@@ -207,7 +382,12 @@ __device__ __forceinline__ void fft_lds(double2* buf, int N, double sign) {
             case 2: radix_pass<2>(buf, N, Ns, sign); break;
             case 3: radix_pass<3>(buf, N, Ns, sign); break;
             case 5: radix_pass<5>(buf, N, Ns, sign); break;
-            default: dft_pass(buf, N, p, Ns, sign); break;
+            default:
+                if (BS && rem == p && use_bluestein(N, p))
+                    bluestein_pass(buf, N, p, sign, tw);
+                else
+                    dft_pass(buf, N, p, Ns, sign);
+                break;
         }
         Ns *= p;
         rem /= p;
@@ -285,9 +465,10 @@ __device__ inline double2 half_spectrum(const double2* buf, double2 xn, int N, i
 }
 
 // ----------------------------------------------------------------------- forward kernels
+template <bool BS>
 __global__ void k_ring_fwd_s0(const double* __restrict__ maps, int64_t map_stride, int nside,
                               int L, int qlo, int qhi, double2* __restrict__ spec, int64_t ld,
-                              int64_t spec_map_stride) {
+                              int64_t spec_map_stride, int dcap) {
     extern __shared__ double2 buf[];
     __shared__ double2 xn;
     const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
@@ -296,7 +477,7 @@ __global__ void k_ring_fwd_s0(const double* __restrict__ maps, int64_t map_strid
     const double* x = maps + blockIdx.y * map_stride + g.offset;
     for (int k = threadIdx.x; k < N; k += blockDim.x) buf[k] = make_double2(x[2 * k], x[2 * k + 1]);
     __syncthreads();
-    fft_lds(buf, N, -1.0);
+    fft_lds<BS>(buf, N, -1.0, buf + dcap);
     r2c_post(buf, N, &xn);
     const double w = 4.0 * M_PI / (12.0 * (double)nside * (double)nside);
     double2* out = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
@@ -307,8 +488,9 @@ __global__ void k_ring_fwd_s0(const double* __restrict__ maps, int64_t map_strid
     }
 }
 
+template <bool BS>
 __global__ void k_ring_fwd_s(const double* __restrict__ q, const double* __restrict__ u, int nside,
-                             int L, int qlo, int qhi, double2* __restrict__ spec, int64_t ld) {
+                             int L, int qlo, int qhi, double2* __restrict__ spec, int64_t ld, int dcap) {
     extern __shared__ double2 buf[];
     __shared__ double2 xn;
     const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
@@ -321,7 +503,7 @@ __global__ void k_ring_fwd_s(const double* __restrict__ q, const double* __restr
         const double* x = (comp == 0 ? q : u) + g.offset;
         for (int k = threadIdx.x; k < N; k += blockDim.x) buf[k] = make_double2(x[2 * k], x[2 * k + 1]);
         __syncthreads();
-        fft_lds(buf, N, -1.0);
+        fft_lds<BS>(buf, N, -1.0, buf + dcap);
         r2c_post(buf, N, &xn);
         for (int j = threadIdx.x; j < W; j += blockDim.x) {
             const int m = j - (L - 1);
@@ -353,9 +535,10 @@ __device__ inline double2 fold_sum(const double2* F, int64_t col0, int mlo, int 
     return acc;
 }
 
+template <bool BS>
 __global__ void k_ring_inv_s0(const double2* __restrict__ spec, int64_t ld, int64_t spec_map_stride,
                               int nside, int L, int qlo, int qhi, double* __restrict__ maps,
-                              int64_t map_stride) {
+                              int64_t map_stride, int dcap) {
     extern __shared__ double2 buf[];
     __shared__ double2 xn;
     const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
@@ -373,7 +556,7 @@ __global__ void k_ring_inv_s0(const double2* __restrict__ spec, int64_t ld, int6
     }
     __syncthreads();
     c2r_pre(buf, N, xn);
-    fft_lds(buf, N, 1.0);
+    fft_lds<BS>(buf, N, 1.0, buf + dcap);
     double* x = maps + blockIdx.y * map_stride + g.offset;
     for (int k = threadIdx.x; k < N; k += blockDim.x) {
         const double2 z = buf[k];
@@ -382,8 +565,9 @@ __global__ void k_ring_inv_s0(const double2* __restrict__ spec, int64_t ld, int6
     }
 }
 
+template <bool BS>
 __global__ void k_ring_inv_s(const double2* __restrict__ spec, int64_t ld, int nside, int L, int qlo,
-                             int qhi, double* __restrict__ q, double* __restrict__ u) {
+                             int qhi, double* __restrict__ q, double* __restrict__ u, int dcap) {
     extern __shared__ double2 buf[];
     __shared__ double2 xn;
     const int r = bucket_ring(nside, qlo, qhi, blockIdx.x);
@@ -409,7 +593,7 @@ __global__ void k_ring_inv_s(const double2* __restrict__ spec, int64_t ld, int n
         }
         __syncthreads();
         c2r_pre(buf, N, xn);
-        fft_lds(buf, N, 1.0);
+        fft_lds<BS>(buf, N, 1.0, buf + dcap);
         double* x = (comp == 0 ? q : u) + g.offset;
         for (int k = threadIdx.x; k < N; k += blockDim.x) {
             const double2 z = buf[k];
@@ -832,24 +1016,51 @@ inline int threads_for(int N) {
 
 constexpr int kMaxNside = 8192;
 
-// Dynamic LDS above 64 KiB needs the kernel's limit raised.  It is raised once per kernel
-// and device, to the largest size any bucket uses, so that launches after the first make
-// no other runtime call (HIP graph capture records launches only).
+// Dynamic LDS of a bucket whose rings hold up to NL points: the next power of two (the
+// Bluestein pass's M-point buffer never exceeds it), at most kLdsN points.
+inline size_t lds_bytes(int NL) {
+    int P = 1;
+    while (P < NL) P <<= 1;
+    if (P > kLdsN) P = kLdsN;
+    if (P < NL) P = NL;
+    return (size_t)P * sizeof(double2);
+}
+
+// Caps whose rings can reach a Bluestein pass (a prime p >= kBluesteinMin with N >= 2 p)
+// run the kernels built with it; they also take the twiddle table (M / 8 + 1 points).
+inline bool bucket_bluestein(const Bucket& b, bool split) { return !split && b.qlo > 0 && b.N >= 2 * kBluesteinMin; }
+inline size_t table_bytes(int dcap) { return (size_t)(dcap / 8 + 1) * sizeof(double2); }
+
+// Dynamic LDS above 64 KiB needs the kernel's limit raised.  It is raised per kernel and
+// device to at least kLdsN points (the largest plain ring buffer) and further when a launch
+// needs more (a Bluestein bucket's twiddle table), so that once every bucket size has been
+// launched no other runtime call is made (HIP graph capture records launches only).
 template <typename K>
 int set_lds(K kernel, size_t bytes) {
     if (bytes <= 64 * 1024) return GM_OK;
-    constexpr size_t kMax = (size_t)kLdsN * sizeof(double2);
+    constexpr size_t kBase = (size_t)kLdsN * sizeof(double2);
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(GM_E_LAUNCH, "hipGetDevice failed");
     static std::mutex mu;
-    static std::vector<std::pair<const void*, int>> done;
+    struct Limit {
+        const void* fn;
+        int dev;
+        size_t bytes;
+    };
+    static std::vector<Limit> done;
     const void* fn = reinterpret_cast<const void*>(kernel);
     std::lock_guard<std::mutex> lock(mu);
-    for (const auto& d : done)
-        if (d.first == fn && d.second == dev) return GM_OK;
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(bytes > kMax ? bytes : kMax));
-    if (e != hipSuccess) return fail(GM_E_LAUNCH, "hipFuncSetAttribute: %s", hipGetErrorString(e));
-    done.emplace_back(fn, dev);
+    Limit* cur = nullptr;
+    for (auto& d : done)
+        if (d.fn == fn && d.dev == dev) cur = &d;
+    if (cur && cur->bytes >= bytes) return GM_OK;
+    const size_t want = bytes > kBase ? bytes : kBase;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)want);
+    if (e != hipSuccess) return fail(GM_E_LAUNCH, "hipFuncSetAttribute(%zu bytes): %s", want, hipGetErrorString(e));
+    if (cur)
+        cur->bytes = want;
+    else
+        done.push_back({fn, dev, want});
     return GM_OK;
 }
 
@@ -881,11 +1092,24 @@ extern "C" int gm_ring_fft_fwd_s0(const double* maps, int64_t nmaps, int64_t map
     for (int i = 0; i < nb; ++i) {
         const bool split = b[i].N > kLdsN;
         const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
-        const size_t lds = (size_t)NL * sizeof(double2);
-        if (int e = set_lds(split ? k_ring_fwd_s0_split : k_ring_fwd_s0, lds)) return e;
-        hipLaunchKernelGGL(split ? k_ring_fwd_s0_split : k_ring_fwd_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
-                           (hipStream_t)stream, maps, map_stride, (int)nside, (int)L, b[i].qlo, b[i].qhi,
-                           reinterpret_cast<double2*>(spec), ld, spec_map_stride);
+        const size_t lds = lds_bytes(NL);
+        const int dcap = (int)(lds / sizeof(double2));
+        const dim3 grid(b[i].nblocks, (unsigned)nmaps), blk(threads_for(b[i].N));
+        double2* sp = reinterpret_cast<double2*>(spec);
+        if (split) {
+            if (int e = set_lds(k_ring_fwd_s0_split, lds)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s0_split, grid, blk, lds, (hipStream_t)stream, maps, map_stride, (int)nside,
+                               (int)L, b[i].qlo, b[i].qhi, sp, ld, spec_map_stride);
+        } else if (bucket_bluestein(b[i], split)) {
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_fwd_s0<true>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s0<true>, grid, blk, lt, (hipStream_t)stream, maps, map_stride, (int)nside,
+                               (int)L, b[i].qlo, b[i].qhi, sp, ld, spec_map_stride, dcap);
+        } else {
+            if (int e = set_lds(k_ring_fwd_s0<false>, lds)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s0<false>, grid, blk, lds, (hipStream_t)stream, maps, map_stride, (int)nside,
+                               (int)L, b[i].qlo, b[i].qhi, sp, ld, spec_map_stride, dcap);
+        }
     }
     return check_launch("gm_ring_fft_fwd_s0");
 }
@@ -901,10 +1125,24 @@ extern "C" int gm_ring_fft_fwd_s(const double* q, const double* u, int64_t nside
     for (int i = 0; i < nb; ++i) {
         const bool split = b[i].N > kLdsN;
         const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
-        const size_t lds = (size_t)NL * sizeof(double2);
-        if (int e = set_lds(split ? k_ring_fwd_s_split : k_ring_fwd_s, lds)) return e;
-        hipLaunchKernelGGL(split ? k_ring_fwd_s_split : k_ring_fwd_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
-                           q, u, (int)nside, (int)L, b[i].qlo, b[i].qhi, reinterpret_cast<double2*>(spec), ld);
+        const size_t lds = lds_bytes(NL);
+        const int dcap = (int)(lds / sizeof(double2));
+        const dim3 grid(b[i].nblocks), blk(threads_for(b[i].N));
+        double2* sp = reinterpret_cast<double2*>(spec);
+        if (split) {
+            if (int e = set_lds(k_ring_fwd_s_split, lds)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s_split, grid, blk, lds, (hipStream_t)stream, q, u, (int)nside, (int)L,
+                               b[i].qlo, b[i].qhi, sp, ld);
+        } else if (bucket_bluestein(b[i], split)) {
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_fwd_s<true>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s<true>, grid, blk, lt, (hipStream_t)stream, q, u, (int)nside, (int)L,
+                               b[i].qlo, b[i].qhi, sp, ld, dcap);
+        } else {
+            if (int e = set_lds(k_ring_fwd_s<false>, lds)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s<false>, grid, blk, lds, (hipStream_t)stream, q, u, (int)nside, (int)L,
+                               b[i].qlo, b[i].qhi, sp, ld, dcap);
+        }
     }
     return check_launch("gm_ring_fft_fwd_s");
 }
@@ -923,11 +1161,24 @@ extern "C" int gm_ring_fft_inv_s0(const double* spec, int64_t nmaps, int64_t ld,
     for (int i = 0; i < nb; ++i) {
         const bool split = b[i].N > kLdsN;
         const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
-        const size_t lds = (size_t)NL * sizeof(double2);
-        if (int e = set_lds(split ? k_ring_inv_s0_split : k_ring_inv_s0, lds)) return e;
-        hipLaunchKernelGGL(split ? k_ring_inv_s0_split : k_ring_inv_s0, dim3(b[i].nblocks, (unsigned)nmaps), dim3(threads_for(b[i].N)), lds,
-                           (hipStream_t)stream, reinterpret_cast<const double2*>(spec), ld, spec_map_stride,
-                           (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride);
+        const size_t lds = lds_bytes(NL);
+        const int dcap = (int)(lds / sizeof(double2));
+        const dim3 grid(b[i].nblocks, (unsigned)nmaps), blk(threads_for(b[i].N));
+        const double2* sp = reinterpret_cast<const double2*>(spec);
+        if (split) {
+            if (int e = set_lds(k_ring_inv_s0_split, lds)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s0_split, grid, blk, lds, (hipStream_t)stream, sp, ld, spec_map_stride,
+                               (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride);
+        } else if (bucket_bluestein(b[i], split)) {
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_inv_s0<true>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s0<true>, grid, blk, lt, (hipStream_t)stream, sp, ld, spec_map_stride,
+                               (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride, dcap);
+        } else {
+            if (int e = set_lds(k_ring_inv_s0<false>, lds)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s0<false>, grid, blk, lds, (hipStream_t)stream, sp, ld, spec_map_stride,
+                               (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride, dcap);
+        }
     }
     return check_launch("gm_ring_fft_inv_s0");
 }
@@ -943,11 +1194,24 @@ extern "C" int gm_ring_fft_inv_s(const double* spec, int64_t ld, int64_t nside, 
     for (int i = 0; i < nb; ++i) {
         const bool split = b[i].N > kLdsN;
         const int NL = split ? b[i].N / 2 : b[i].N;  // points held in LDS
-        const size_t lds = (size_t)NL * sizeof(double2);
-        if (int e = set_lds(split ? k_ring_inv_s_split : k_ring_inv_s, lds)) return e;
-        hipLaunchKernelGGL(split ? k_ring_inv_s_split : k_ring_inv_s, dim3(b[i].nblocks), dim3(threads_for(b[i].N)), lds, (hipStream_t)stream,
-                           reinterpret_cast<const double2*>(spec), ld, (int)nside, (int)L, b[i].qlo, b[i].qhi,
-                           q, u);
+        const size_t lds = lds_bytes(NL);
+        const int dcap = (int)(lds / sizeof(double2));
+        const dim3 grid(b[i].nblocks), blk(threads_for(b[i].N));
+        const double2* sp = reinterpret_cast<const double2*>(spec);
+        if (split) {
+            if (int e = set_lds(k_ring_inv_s_split, lds)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s_split, grid, blk, lds, (hipStream_t)stream, sp, ld, (int)nside, (int)L,
+                               b[i].qlo, b[i].qhi, q, u);
+        } else if (bucket_bluestein(b[i], split)) {
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_inv_s<true>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s<true>, grid, blk, lt, (hipStream_t)stream, sp, ld, (int)nside, (int)L,
+                               b[i].qlo, b[i].qhi, q, u, dcap);
+        } else {
+            if (int e = set_lds(k_ring_inv_s<false>, lds)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s<false>, grid, blk, lds, (hipStream_t)stream, sp, ld, (int)nside, (int)L,
+                               b[i].qlo, b[i].qhi, q, u, dcap);
+        }
     }
     return check_launch("gm_ring_fft_inv_s");
 }
