@@ -524,15 +524,22 @@ __global__ void k_ring_fwd_s(const double* __restrict__ q, const double* __restr
 
 // ----------------------------------------------------------------------- inverse kernels
 // Y(k) = sum_{m = k mod n, m in [mlo, mhi]} F[m] e^{i m phi0}; F indexed at m - mlo_index.
+// phi0 = pi num / n on every ring (num = 0 or 1), so the aliases m0 + j n of one k share the
+// phase of m0 up to the sign (-1)^(j num): one phase per k.
 __device__ inline double2 fold_sum(const double2* F, int64_t col0, int mlo, int mhi, int k, int n,
                                    const Ring& g) {
     // first m >= mlo with m = k (mod n)
     int64_t t = ((int64_t)k - mlo) % n;
     if (t < 0) t += n;
+    const int64_t m0 = mlo + t;
     double2 acc = make_double2(0.0, 0.0);
-    for (int64_t m = mlo + t; m <= mhi; m += n)
-        acc = cadd(acc, cmul(F[m + col0], phase_pi(m, g.phi0_num, g.phi0_den, 1.0)));
-    return acc;
+    bool neg = false;
+    for (int64_t m = m0; m <= mhi; m += n) {
+        const double2 v = F[m + col0];
+        acc = neg ? csub(acc, v) : cadd(acc, v);
+        if (g.phi0_num) neg = !neg;
+    }
+    return cmul(acc, phase_pi(m0, g.phi0_num, g.phi0_den, 1.0));
 }
 
 template <bool BS>
