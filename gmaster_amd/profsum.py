@@ -16,12 +16,13 @@ import re
 import sys
 from collections import defaultdict
 
-_KERNELS = ("k_bat_ana_h", "k_bat_ana", "k_bat_syn64", "k_bat_syn", "k_bat_finish", "k_bat_prep", "k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
+_KERNELS = ("k_coupling_pair_values", "k_toeplitz_correct", "k_bat_ana_h", "k_bat_ana", "k_bat_syn64", "k_bat_syn", "k_bat_finish", "k_bat_prep", "k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
             "k_coupling", "k_toeplitz", "k_ring_fwd_s0", "k_ring_fwd_s", "k_ring_inv_s0", "k_ring_inv_s", "k_fold_s0", "k_fold_s",
             "k_finish_s0", "k_finish_s", "k_prep_s0", "k_prep_s", "k_lat_tables", "k_cl_partial", "k_cl_final",
             "k_bin_block", "k_bin_cell", "k_gauss_legendre", "k_wigner_d_table", "k_wigner_d_corr",
             "k_table_combine", "k_parity_split", "k_ring_table", "k_masked_links", "k_apodize", "k_gaussian_alms",
-            "k_spin_mul", "k_alm_filter", "k_dot_partial", "k_dot_final", "k_deproject")
+            "k_spin_mul", "k_alm_filter", "k_dot_partial", "k_dot_final", "k_deproject", "k_lu_clear", "k_lu_diag",
+            "k_lu_panel", "k_lu_update", "k_lu_solve", "k_coupling_pair_values", "k_toeplitz_correct", "k_fingerprint")
 
 
 def short(name):
@@ -136,10 +137,13 @@ def traffic_from_summaries(fetch_txt, write_txt, kernel, copy_bytes):
         hits = [r for r in rows if _is(short(r[0]), kernel)]
         if not hits:
             raise SystemExit(f"kernel {kernel} not found in {path}")
-        name, n, mean_kib, _ = hits[0]
+        # a bare name covers every instantiation: dispatch-weighted mean
+        n = sum(h[1] for h in hits)
+        mean_kib = sum(h[1] * h[2] for h in hits) / n
+        name = kernel if len(hits) > 1 else short(hits[0][0])
         scale = copy_bytes / (ref_kib * 1024.0)
         out[tag] = mean_kib * 1024.0 * scale
-        print(f"{counter}: {short(name)} mean {mean_kib:.1f} KiB over {n} dispatches; copy kernel "
+        print(f"{counter}: {name} mean {mean_kib:.1f} KiB over {n} dispatches; copy kernel "
               f"{ref_kib:.1f} KiB for {copy_bytes / 2**30:.3f} GiB -> scale {scale:.4f}; "
               f"scaled {out[tag] / 1e9:.3f} GB per launch")
     print(f"traffic_bytes_per_launch: {out['fetch'] + out['write']:.6e}")
