@@ -54,6 +54,7 @@ SIGNATURES = {
     "gm_coupled_cell_noalloc_max_work": [],
     "gm_coupling": [_p, _i64, _p, _i64, _p, _p, _p, _p, _p],
     "gm_coupling_pairs": [_p, _i64, _p, _i64, _p, _p, _i64, _p, _p, _p, _p, _p],
+    "gm_coupling_rows": [_p, _i64, _p, _i64, _p, _i64, _p, _p, _p, _p, _p],
     "gm_mlp_fwd": [_p, _p, _p, _i64, _i64, _i64, _i32, _p, _p, _p, _p],
     "gm_mlp_bwd": [_p, _p, _i32, _p, _p, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p],
     "gm_to_bf16": [_p, _i64, _p, _p],

@@ -67,9 +67,29 @@ __global__ __launch_bounds__(256) void k_coupling(const double* __restrict__ Wp,
     j = __builtin_amdgcn_readfirstlane(j);
     LdsWin win{win_buf[wave], Wp, lmax_w, 64 * j, lane};
     double s[4];
-    couple_wave_lane(win, lmax_w, g, L, l1, j, lane, s);
+    couple_wave_lane(win, lmax_w, g, L, l1, l1 + 64 * j, lane, s);
     const int l2 = l1 + 64 * j + lane;
     if (l2 < L) store_pair(l1, l2, L, s, m00, m02, m22p, m22m);
+}
+
+// Listed row segments, one wave per item: items[3 k .. 3 k + 2] = (l1, l2_0, nlive), the
+// pairs (l1, l2_0 + i) for i < nlive <= 64 (l1 <= l2_0, l2_0 + nlive <= L).
+__global__ __launch_bounds__(256) void k_coupling_rows(const double* __restrict__ Wp, int lmax_w,
+                                                       const double* __restrict__ g, int L,
+                                                       const int32_t* __restrict__ items, int64_t nitems,
+                                                       double* __restrict__ m00, double* __restrict__ m02,
+                                                       double* __restrict__ m22p, double* __restrict__ m22m) {
+    __shared__ double win_buf[4][128];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(wave);
+    if (item >= nitems) return;
+    const int l1 = __builtin_amdgcn_readfirstlane(items[3 * item]);
+    const int l2_0 = __builtin_amdgcn_readfirstlane(items[3 * item + 1]);
+    const int nlive = __builtin_amdgcn_readfirstlane(items[3 * item + 2]);
+    LdsWin win{win_buf[wave], Wp, lmax_w, l2_0 - l1, lane};
+    double s[4];
+    couple_wave_lane(win, lmax_w, g, L, l1, l2_0, lane, s);
+    if (lane < nlive) store_pair(l1, l2_0 + lane, L, s, m00, m02, m22p, m22m);
 }
 
 // The four row sums (without the column factor) of each listed pair, vals[k][0..3].
@@ -272,6 +292,21 @@ extern "C" int gm_coupling(const double* window, int64_t lmax_window, const doub
     hipLaunchKernelGGL(k_coupling, dim3((unsigned)((items + 3) / 4)), dim3(256), 0, (hipStream_t)stream, window,
                        (int)lmax_window, g, L, items, m00, m02, m22p, m22m);
     return check_launch("gm_coupling");
+}
+
+extern "C" int gm_coupling_rows(const double* window, int64_t lmax_window, const double* g, int64_t lmax,
+                                const int32_t* items, int64_t nitems, double* m00, double* m02, double* m22p,
+                                double* m22m, void* stream) {
+    if (!window || !g || (nitems > 0 && !items)) return fail(GM_E_ARG, "gm_coupling_rows: null pointer");
+    if (lmax < 0 || lmax > 65535) return fail(GM_E_ARG, "gm_coupling_rows: bad lmax %lld", (long long)lmax);
+    if (!m00 && !m02 && !m22p && !m22m) return fail(GM_E_ARG, "gm_coupling_rows: no output requested");
+    if (lmax_window < 0 || lmax_window > 2 * lmax)
+        return fail(GM_E_ARG, "gm_coupling_rows: need 0 <= lmax_window <= 2 lmax");
+    if (nitems < 0) return fail(GM_E_ARG, "gm_coupling_rows: negative item count");
+    if (nitems == 0) return GM_OK;
+    hipLaunchKernelGGL(k_coupling_rows, dim3((unsigned)((nitems + 3) / 4)), dim3(256), 0, (hipStream_t)stream, window,
+                       (int)lmax_window, g, (int)lmax + 1, items, nitems, m00, m02, m22p, m22m);
+    return check_launch("gm_coupling_rows");
 }
 
 extern "C" int gm_coupling_pairs(const double* window, int64_t lmax_window, const double* g, int64_t lmax,

@@ -168,17 +168,17 @@ __host__ __device__ inline void couple_decode(int64_t item, int L, int& l1, int&
     j = (int)(r % (t + 1));
 }
 
-// One lane of a wave's item.  All lanes run the degree count of lane 0 (the smallest d,
-// so the longest row): a lane with fewer degrees in the window reads zeros past lmax_w
-// (the window is zero-padded), and every lane stays inside its triangle (l3 <= l1 + l2).
-// Win supplies W' per lane: stage(k0) loads W'[64 j + k0 + t], t < 128 (zero past lmax_w),
-// and at(t) returns entry t; lane i at degree l3 = d + k reads entry i + k - k0, with
-// k0 = 64 floor(k / 64).  Lanes with l2 >= L run the row of l2 = L - 1 and are not stored.
+// One lane of a wave's item: row l1, columns l2 = l2_0 + lane (l2_0 >= l1).  All lanes run
+// the degree count of lane 0 (the smallest d, so the longest row): a lane with fewer
+// degrees in the window reads zeros past lmax_w (the window is zero-padded), and every lane
+// stays inside its triangle (l3 <= l1 + l2).  Win supplies W' per lane: stage(k0) loads
+// W'[d0 + k0 + t], t < 128, d0 = l2_0 - l1 (zero past lmax_w), and at(t) returns entry t;
+// lane i at degree l3 = d0 + i + k reads entry i + k - k0, with k0 = 64 floor(k / 64).
+// Lanes with l2 >= L run the row of l2 = L - 1; the caller stores only the lanes it owns.
 template <class Win>
 __host__ __device__ inline void couple_wave_lane(Win& win, int lmax_w, const double* __restrict__ g, int L, int l1,
-                                                 int j, int lane, double s[4]) {
-    const int l2_0 = l1 + 64 * j;
-    const int d0 = 64 * j;
+                                                 int l2_0, int lane, double s[4]) {
+    const int d0 = l2_0 - l1;
     const int n = min(l1 + l2_0, lmax_w) - d0 + 1;  // lane 0's degrees (all lanes run these)
     const int l2 = min(l2_0 + lane, L - 1);
     if (n <= 0) {

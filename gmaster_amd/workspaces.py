@@ -86,6 +86,35 @@ def _toeplitz_pairs(L, l_toeplitz, l_exact, dl_band, device):
     return lo.to(torch.int32), hi.to(torch.int32)
 
 
+def _toeplitz_items(L, l_toeplitz, l_exact, dl_band, device):
+    """``gm_coupling_rows`` items ``(n, 3)`` int32 covering exactly the pairs of
+    ``_toeplitz_pairs``: row segments (rows lo <= l_exact, the band rows, row l_toeplitz past
+    its band) cut into 64-column waves, and one single-pair item per entry of column
+    l_toeplitz above the band."""
+    lt, le, dl = int(l_toeplitz), int(l_exact), int(dl_band)
+    i64 = dict(dtype=torch.int64, device=device)
+    ra = torch.arange(0, min(le, L - 1) + 1, **i64)
+    rb = torch.arange(le + 1, L, **i64)
+    rows = [ra, rb]
+    starts = [ra, rb]
+    counts = [L - ra, torch.clamp(L - rb, max=dl + 1)]
+    if lt > le and lt + dl + 1 < L:
+        rows.append(torch.tensor([lt], **i64))
+        starts.append(torch.tensor([lt + dl + 1], **i64))
+        counts.append(torch.tensor([L - (lt + dl + 1)], **i64))
+    c1 = torch.arange(le + 1, max(le + 1, lt - dl), **i64)  # (lo, lt) above the band
+    rows.append(c1)
+    starts.append(torch.full_like(c1, lt))
+    counts.append(torch.ones_like(c1))
+    row, start, count = torch.cat(rows), torch.cat(starts), torch.cat(counts)
+    keep = count > 0
+    row, start, count = row[keep], start[keep], count[keep]
+    nch = (count + 63) // 64
+    seg, k = _runs(torch.zeros_like(nch), nch, device)  # chunk k of segment seg
+    items = torch.stack([row[seg], start[seg] + 64 * k, torch.clamp(count[seg] - 64 * k, max=64)], 1)
+    return items.to(torch.int32).contiguous()
+
+
 def coupling_blocks(window, lmax, toeplitz=None):
     """``{"00", "02", "22p", "22m"}`` coupling blocks ``(lmax+1, lmax+1)`` for a mask window.
 
@@ -114,10 +143,10 @@ def coupling_blocks(window, lmax, toeplitz=None):
                   stream)
     else:
         lt, le, dl = tkey
-        lo, hi = _toeplitz_pairs(L, lt, le, dl, dev)
+        items = _toeplitz_items(L, lt, le, dl, dev)
         exact = {k: torch.zeros((L, L), dtype=torch.float64, device=dev) for k in names}
-        _lib.call("gm_coupling_pairs", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(lo), _lib.ptr(hi),
-                  lo.numel(), *(_lib.ptr(exact[k]) for k in names), stream)
+        _lib.call("gm_coupling_rows", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(items),
+                  items.shape[0], *(_lib.ptr(exact[k]) for k in names), stream)
         blocks = {}
         for k in names:
             blocks[k] = torch.empty((L, L), dtype=torch.float64, device=dev)

@@ -233,6 +233,16 @@ int gm_coupling_pairs(const double* window, int64_t lmax_window, const double* g
                       const int32_t* lo, const int32_t* hi, int64_t npairs, double* m00, double* m02,
                       double* m22p, double* m22m, void* stream);
 
+/* gm_coupling for listed row segments: items (int32 triples, nitems of them) give (l1, l2_0,
+ * nlive), the pairs (l1, l2_0 + i) for i < nlive <= 64, with 0 <= l1 <= l2_0 and
+ * l2_0 + nlive <= lmax + 1 (not checked on the device); one wavefront per item, the same
+ * recursion and window staging as gm_coupling.  Writes both (l1, l2) and (l2, l1) and
+ * leaves every other element alone.  Replaces the exact-entry evaluation of
+ * gmaster/workspaces.py:542-590 when the entries come in rows, as the Toeplitz fill's do. */
+int gm_coupling_rows(const double* window, int64_t lmax_window, const double* g, int64_t lmax,
+                     const int32_t* items, int64_t nitems, double* m00, double* m02, double* m22p,
+                     double* m22m, void* stream);
+
 /* Toeplitz approximation of one coupling block M ((lmax+1)^2, column factor included)
  * into out (not aliasing M): NaMaster's fill with l_toeplitz, l_exact, dl_band
  * (0 < l_exact <= l_toeplitz < lmax).  Replaces gmaster/workspaces.py:1141-1194

@@ -59,7 +59,7 @@ int main(int argc, char** argv) {
         for (int lane = 0; lane < 64; ++lane) {
             HostWin w{wp.data(), lmax_w, 64 * j, 0, false};
             double s[4];
-            gm::couple_wave_lane(w, lmax_w, g.data(), L, l1, j, lane, s);
+            gm::couple_wave_lane(w, lmax_w, g.data(), L, l1, l1 + 64 * j, lane, s);
             if (w.bad) status = 3;
             const int l2 = l1 + 64 * j + lane;
             if (l2 >= L) continue;

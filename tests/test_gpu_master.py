@@ -171,6 +171,43 @@ def test_coupling_pairs_match_full_kernel():
         assert bool(torch.isnan(out[k][~sel]).all()), k
 
 
+@pytest.mark.parametrize("lmax,lw", [(300, 600), (300, 300), (700, 700)])
+def test_coupling_rows_match_full_kernel(lmax, lw):
+    """gm_coupling_rows over the Toeplitz fill's row items (the workspace path) against the
+    full kernel: bit for bit where the window reaches 2 lmax (every lane of a wave runs the
+    same degrees either way), to 1e-14 of the block where it stops at lmax (lanes cut short
+    take a few more zero-weight steps than in the full kernel's waves); untouched elsewhere."""
+    from gmaster_amd import _lib
+    from gmaster_amd.workspaces import _g_table, _toeplitz_items, clear_coupling_cache, coupling_blocks
+
+    clear_coupling_cache()
+    L = lmax + 1
+    window = np.random.default_rng(lmax + lw).uniform(size=lw + 1) / (1.0 + np.arange(lw + 1))
+    w = torch.as_tensor(window, device="cuda")
+    full = coupling_blocks(w, lmax)
+    wp = (w * (2 * torch.arange(len(w), dtype=torch.float64, device="cuda") + 1) / (4 * np.pi)).contiguous()
+    lt, le, dl = lmax // 2, lmax // 10, lmax // 8
+    items = _toeplitz_items(L, lt, le, dl, "cuda")
+    print(f"coupling rows lmax={lmax} lw={lw}: {items.shape[0]} items", flush=True)
+    out = {k: torch.full((L, L), float("nan"), dtype=torch.float64, device="cuda") for k in ("00", "02", "22p", "22m")}
+    _lib.call("gm_coupling_rows", _lib.ptr(wp), len(w) - 1, _lib.ptr(_g_table(L + 1, "cuda")), lmax, _lib.ptr(items),
+              items.shape[0], *(_lib.ptr(out[k]) for k in ("00", "02", "22p", "22m")), _lib.stream_ptr("cuda"))
+    torch.cuda.synchronize()
+    it = items.long()
+    sel = torch.zeros((L, L), dtype=torch.bool, device="cuda")
+    for k in range(64):
+        live = it[:, 2] > k
+        sel[it[live, 0], it[live, 1] + k] = True
+    sel = sel | sel.T
+    for k in out:
+        if lw >= 2 * lmax:
+            assert torch.equal(out[k][sel], full[k][sel]), k
+        else:
+            err = float((out[k][sel] - full[k][sel]).abs().max() / full[k].abs().max())
+            assert err < 1e-14, (k, err)
+        assert bool(torch.isnan(out[k][~sel]).all()), k
+
+
 @pytest.mark.parametrize("lt,le,dl", TOEPLITZ)
 def test_toeplitz_kernel_matches_oracle(lt, le, dl):
     from gmaster_amd import _lib
