@@ -365,10 +365,17 @@ __device__ int smallest_factor(int n) {
 // the FMA Horner step above, it returned spectra off by up to ~1e-7 relative on gfx950
 // (rings that never reach the direct pass included), while the inlined form matches
 // numpy to ~1e-16; the cause was not isolated.
-// BS: prime passes >= kBluesteinMin may run as a Bluestein convolution (tw: its twiddle
-// table in LDS, M / 8 + 1 points past the data buffer).
+// tw: LDS for a twiddle table of N / 8 + 1 points past the data buffer (or null); with it,
+// power-of-two lengths take their twiddles from the table.  BS: prime passes
+// >= kBluesteinMin may run as a Bluestein convolution (its table goes there too).
 template <bool BS = false>
 __device__ __forceinline__ void fft_lds(double2* buf, int N, double sign, double2* tw = nullptr) {
+    if (tw != nullptr && N >= 8 && (N & (N - 1)) == 0) {  // power of two: twiddles from an LDS table
+        for (int k = threadIdx.x; k <= N / 8; k += blockDim.x) tw[k] = twiddle(k, N, -1.0);
+        __syncthreads();
+        fft_pow2(buf, N, sign, tw);
+        return;
+    }
     int Ns = 1, rem = N;
     while (rem > 1) {
         int p;
@@ -481,10 +488,13 @@ __global__ void k_ring_fwd_s0(const double* __restrict__ maps, int64_t map_strid
     r2c_post(buf, N, &xn);
     const double w = 4.0 * M_PI / (12.0 * (double)nside * (double)nside);
     double2* out = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
+    // w e^{-i m phi0}, advanced by the fixed rotation of a thread's stride
+    double2 ph = cscale(phase_pi(threadIdx.x, g.phi0_num, g.phi0_den, -1.0), w);
+    const double2 rot = phase_pi(blockDim.x, g.phi0_num, g.phi0_den, -1.0);
     for (int m = threadIdx.x; m < L; m += blockDim.x) {
         const double2 X = half_spectrum(buf, xn, N, m % n);
-        const double2 ph = phase_pi(m, g.phi0_num, g.phi0_den, -1.0);
-        out[m] = cscale(cmul(ph, X), w);
+        out[m] = cmul(ph, X);
+        ph = cmul(ph, rot);
     }
 }
 
@@ -505,12 +515,15 @@ __global__ void k_ring_fwd_s(const double* __restrict__ q, const double* __restr
         __syncthreads();
         fft_lds<BS>(buf, N, -1.0, buf + dcap);
         r2c_post(buf, N, &xn);
+        double2 ph = cscale(phase_pi((int)threadIdx.x - (L - 1), g.phi0_num, g.phi0_den, -1.0), w);
+        const double2 rot = phase_pi(blockDim.x, g.phi0_num, g.phi0_den, -1.0);
         for (int j = threadIdx.x; j < W; j += blockDim.x) {
             const int m = j - (L - 1);
             int kk = m % n;
             if (kk < 0) kk += n;
             const double2 X = half_spectrum(buf, xn, N, kk);
-            const double2 v = cscale(cmul(phase_pi(m, g.phi0_num, g.phi0_den, -1.0), X), w);
+            const double2 v = cmul(ph, X);
+            ph = cmul(ph, rot);
             if (comp == 0) {
                 out[j] = v;
             } else {
@@ -712,9 +725,12 @@ __global__ void k_ring_fwd_s0_split(const double* __restrict__ maps, int64_t map
         load_half(buf, x, H, p);
         fft_lds(buf, H, -1.0);
         r2c_post_half(buf, H, p, &xn);
+        double2 ph = cscale(phase_pi(p + 2 * (int)threadIdx.x, g.phi0_num, g.phi0_den, -1.0), w);
+        const double2 rot = phase_pi(2 * blockDim.x, g.phi0_num, g.phi0_den, -1.0);
         for (int m = p + 2 * threadIdx.x; m < L; m += 2 * blockDim.x) {
             const double2 X = half_parity_x(buf, xn, N, p, m % n);
-            out[m] = cscale(cmul(phase_pi(m, g.phi0_num, g.phi0_den, -1.0), X), w);
+            out[m] = cmul(ph, X);
+            ph = cmul(ph, rot);
         }
         __syncthreads();
     }
@@ -737,12 +753,16 @@ __global__ void k_ring_fwd_s_split(const double* __restrict__ q, const double* _
             fft_lds(buf, H, -1.0);
             r2c_post_half(buf, H, p, &xn);
             // outputs m = j - (L - 1) of parity p; the same thread owns j in both components
-            for (int j = ((L - 1 + p) & 1) + 2 * threadIdx.x; j < W; j += 2 * blockDim.x) {
+            const int j0 = ((L - 1 + p) & 1) + 2 * (int)threadIdx.x;
+            double2 ph = cscale(phase_pi(j0 - (L - 1), g.phi0_num, g.phi0_den, -1.0), w);
+            const double2 rot = phase_pi(2 * blockDim.x, g.phi0_num, g.phi0_den, -1.0);
+            for (int j = j0; j < W; j += 2 * blockDim.x) {
                 const int m = j - (L - 1);
                 int kk = m % n;
                 if (kk < 0) kk += n;
                 const double2 X = half_parity_x(buf, xn, N, p, kk);
-                const double2 v = cscale(cmul(phase_pi(m, g.phi0_num, g.phi0_den, -1.0), X), w);
+                const double2 v = cmul(ph, X);
+                ph = cmul(ph, rot);
                 if (comp == 0) {
                     out[j] = v;
                 } else {
@@ -1034,7 +1054,8 @@ inline size_t lds_bytes(int NL) {
 }
 
 // Caps whose rings can reach a Bluestein pass (a prime p >= kBluesteinMin with N >= 2 p)
-// run the kernels built with it; they also take the twiddle table (M / 8 + 1 points).
+// run the kernels built with it.  Every non-split launch carries the twiddle table
+// (dcap / 8 + 1 points: power-of-two rings and Bluestein convolutions use it).
 inline bool bucket_bluestein(const Bucket& b, bool split) { return !split && b.qlo > 0 && b.N >= 2 * kBluesteinMin; }
 inline size_t table_bytes(int dcap) { return (size_t)(dcap / 8 + 1) * sizeof(double2); }
 
@@ -1113,8 +1134,9 @@ extern "C" int gm_ring_fft_fwd_s0(const double* maps, int64_t nmaps, int64_t map
             hipLaunchKernelGGL(k_ring_fwd_s0<true>, grid, blk, lt, (hipStream_t)stream, maps, map_stride, (int)nside,
                                (int)L, b[i].qlo, b[i].qhi, sp, ld, spec_map_stride, dcap);
         } else {
-            if (int e = set_lds(k_ring_fwd_s0<false>, lds)) return e;
-            hipLaunchKernelGGL(k_ring_fwd_s0<false>, grid, blk, lds, (hipStream_t)stream, maps, map_stride, (int)nside,
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_fwd_s0<false>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s0<false>, grid, blk, lt, (hipStream_t)stream, maps, map_stride, (int)nside,
                                (int)L, b[i].qlo, b[i].qhi, sp, ld, spec_map_stride, dcap);
         }
     }
@@ -1146,8 +1168,9 @@ extern "C" int gm_ring_fft_fwd_s(const double* q, const double* u, int64_t nside
             hipLaunchKernelGGL(k_ring_fwd_s<true>, grid, blk, lt, (hipStream_t)stream, q, u, (int)nside, (int)L,
                                b[i].qlo, b[i].qhi, sp, ld, dcap);
         } else {
-            if (int e = set_lds(k_ring_fwd_s<false>, lds)) return e;
-            hipLaunchKernelGGL(k_ring_fwd_s<false>, grid, blk, lds, (hipStream_t)stream, q, u, (int)nside, (int)L,
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_fwd_s<false>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_fwd_s<false>, grid, blk, lt, (hipStream_t)stream, q, u, (int)nside, (int)L,
                                b[i].qlo, b[i].qhi, sp, ld, dcap);
         }
     }
@@ -1182,8 +1205,9 @@ extern "C" int gm_ring_fft_inv_s0(const double* spec, int64_t nmaps, int64_t ld,
             hipLaunchKernelGGL(k_ring_inv_s0<true>, grid, blk, lt, (hipStream_t)stream, sp, ld, spec_map_stride,
                                (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride, dcap);
         } else {
-            if (int e = set_lds(k_ring_inv_s0<false>, lds)) return e;
-            hipLaunchKernelGGL(k_ring_inv_s0<false>, grid, blk, lds, (hipStream_t)stream, sp, ld, spec_map_stride,
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_inv_s0<false>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s0<false>, grid, blk, lt, (hipStream_t)stream, sp, ld, spec_map_stride,
                                (int)nside, (int)L, b[i].qlo, b[i].qhi, maps, map_stride, dcap);
         }
     }
@@ -1215,8 +1239,9 @@ extern "C" int gm_ring_fft_inv_s(const double* spec, int64_t ld, int64_t nside, 
             hipLaunchKernelGGL(k_ring_inv_s<true>, grid, blk, lt, (hipStream_t)stream, sp, ld, (int)nside, (int)L,
                                b[i].qlo, b[i].qhi, q, u, dcap);
         } else {
-            if (int e = set_lds(k_ring_inv_s<false>, lds)) return e;
-            hipLaunchKernelGGL(k_ring_inv_s<false>, grid, blk, lds, (hipStream_t)stream, sp, ld, (int)nside, (int)L,
+            const size_t lt = lds + table_bytes(dcap);
+            if (int e = set_lds(k_ring_inv_s<false>, lt)) return e;
+            hipLaunchKernelGGL(k_ring_inv_s<false>, grid, blk, lt, (hipStream_t)stream, sp, ld, (int)nside, (int)L,
                                b[i].qlo, b[i].qhi, q, u, dcap);
         }
     }
