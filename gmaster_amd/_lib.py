@@ -64,7 +64,7 @@ SIGNATURES = {
     "gm_toeplitz_correct": [_p, _i64, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _p],
     "gm_toeplitz": [_p, _i64, _i64, _i64, _i64, _p, _p],
     "gm_lu_factor": [_p, _i64, _p, _p],
-    "gm_lu_solve": [_p, _i64, _p, _i64, _p],
+    "gm_lu_solve": [_p, _p, _i64, _p, _i64, _p],
     "gm_bin_block": [_p, _i64, _p, _p, _i64, _p, _p, _p, _p],
     "gm_bin_cell": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _p],
     "gm_fingerprint": [_p, _i64, _p, _p],

@@ -18,7 +18,7 @@
 //                staged in LDS; each wave a 32 x 32 quarter, 2 x 2 MFMA blocks).
 // gm_lu_solve: B <- U^-1 L^-1 B for row-major B (n x nrhs), one workgroup per group of
 // kSolveR right-hand sides, 64-row blocks: the block's diagonal system in LDS, then the
-// rows below (above) updated in parallel.
+// rows below (above) updated in parallel from the transposed factors (coalesced).
 #include "common.h"
 
 namespace gm {
@@ -199,8 +199,8 @@ __global__ __launch_bounds__(256) void k_lu_update(double* __restrict__ A, int n
 // B <- L^-1 B then U^-1 B for the right-hand sides kSolveR * blockIdx.x .. + kSolveR - 1.
 // Per 64-row block: wave q solves the block's triangular system for right-hand side q with
 // lane s holding row s (one shuffle per row), then all threads update the remaining rows.
-__global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ LU, int n, double* __restrict__ B,
-                                                   int nrhs) {
+__global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ LU, const double* __restrict__ LUT, int n,
+                                                   double* __restrict__ B, int nrhs) {
     __shared__ double T[kNB][kNB + 1];
     __shared__ double X[kNB][kSolveR];
     const int tid = threadIdx.x;
@@ -214,14 +214,16 @@ __global__ __launch_bounds__(1024) void k_lu_solve(const double* __restrict__ LU
             T[r][c] = (r < b && c < b) ? LU[(int64_t)(j0 + r) * n + j0 + c] : 0.0;
         }
     };
-    auto update = [&](int lo, int hi, int j0, int b) {  // B[i] -= LU[i][j0 .. j0 + b) X for lo <= i < hi
+    // B[i] -= LU[i][j0 .. j0 + b) X for lo <= i < hi, reading the transposed factors so that
+    // consecutive threads (rows) read consecutive words
+    auto update = [&](int lo, int hi, int j0, int b) {
         for (int i = lo + tid; i < hi; i += 1024) {
-            const double* row = LU + (int64_t)i * n + j0;
+            const double* col = LUT + (int64_t)j0 * n + i;
             double acc[kSolveR];
 #pragma unroll
             for (int q = 0; q < kSolveR; ++q) acc[q] = 0.0;
             for (int c = 0; c < b; ++c) {
-                const double v = row[c];
+                const double v = col[(int64_t)c * n];
 #pragma unroll
                 for (int q = 0; q < kSolveR; ++q) acc[q] = fma(v, X[c][q], acc[q]);
             }
@@ -294,11 +296,11 @@ extern "C" int gm_lu_factor(double* A, int64_t n, int32_t* flag, void* stream) {
     return check_launch("gm_lu_factor");
 }
 
-extern "C" int gm_lu_solve(const double* LU, int64_t n, double* B, int64_t nrhs, void* stream) {
-    if (!LU || !B) return fail(GM_E_ARG, "gm_lu_solve: null pointer");
+extern "C" int gm_lu_solve(const double* LU, const double* LUT, int64_t n, double* B, int64_t nrhs, void* stream) {
+    if (!LU || !LUT || !B) return fail(GM_E_ARG, "gm_lu_solve: null pointer");
     if (n < 1 || n > (1 << 20) || nrhs < 1 || nrhs > (1 << 26))
         return fail(GM_E_ARG, "gm_lu_solve: bad sizes n=%lld nrhs=%lld", (long long)n, (long long)nrhs);
     hipLaunchKernelGGL(k_lu_solve, dim3((unsigned)((nrhs + kSolveR - 1) / kSolveR)), dim3(1024), 0,
-                       (hipStream_t)stream, LU, (int)n, B, (int)nrhs);
+                       (hipStream_t)stream, LU, LUT, (int)n, B, (int)nrhs);
     return check_launch("gm_lu_solve");
 }

@@ -358,12 +358,12 @@ def lu_factor(M):
     _lib.call("gm_lu_factor", _lib.ptr(lu), n, _lib.ptr(flag), _lib.stream_ptr(M.device))
     if int(flag.item()) != 0:
         return ("pivoted", torch.linalg.lu_factor(M))
-    return ("plain", lu)
+    return ("plain", lu, lu.T.contiguous())
 
 
 def lu_solve(fac, rhs):
     """``M^-1 rhs`` for ``fac = lu_factor(M)`` and ``rhs`` ``(n, k)``."""
-    kind, f = fac
+    kind, f = fac[0], fac[1]
     if kind == "pivoted":
         return torch.linalg.lu_solve(*f, rhs)
     n = f.shape[0]
@@ -371,7 +371,7 @@ def lu_solve(fac, rhs):
         piv = torch.arange(1, n + 1, dtype=torch.int32, device=f.device)
         return torch.linalg.lu_solve(f, piv, rhs)
     out = rhs.to(torch.float64).contiguous().clone()
-    _lib.call("gm_lu_solve", _lib.ptr(f), n, _lib.ptr(out), out.shape[1], _lib.stream_ptr(f.device))
+    _lib.call("gm_lu_solve", _lib.ptr(f), _lib.ptr(fac[2]), n, _lib.ptr(out), out.shape[1], _lib.stream_ptr(f.device))
     return out
 
 

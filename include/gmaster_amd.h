@@ -257,9 +257,10 @@ int gm_toeplitz(const double* M, int64_t lmax, int64_t l_toeplitz, int64_t l_exa
  * inside jnp.linalg.solve(mcm_binned, ...) of gmaster/workspaces.py:2146 (decouple_cell). */
 int gm_lu_factor(double* A, int64_t n, int32_t* flag, void* stream);
 
-/* B <- U^-1 L^-1 B for the factors of gm_lu_factor and row-major B (n x nrhs), in place.
- * Replaces the triangular solves of the same jnp.linalg.solve. */
-int gm_lu_solve(const double* LU, int64_t n, double* B, int64_t nrhs, void* stream);
+/* B <- U^-1 L^-1 B for the factors of gm_lu_factor and row-major B (n x nrhs), in place;
+ * LUT is the transpose of LU (n x n, row-major), which the row updates read.  Replaces the
+ * triangular solves of the same jnp.linalg.solve. */
+int gm_lu_solve(const double* LU, const double* LUT, int64_t n, double* B, int64_t nrhs, void* stream);
 
 /* Binned block: out[b1][b2] = sum_{l1 in b1, l2 in b2} wout[l1] M[l1][l2] tin[l2]
  * (CSR bands: ells sorted by band, band_start has nbands+1 entries).
