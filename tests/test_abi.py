@@ -98,6 +98,14 @@ def dummy_pointer():
         ("gm_coupled_cell_noalloc", (1, 2, 1, 2, 1 << 26, 9000, 1, None), 2),  # partials beyond the 64 MiB array
         ("gm_coupling_pairs", (1, 20, 1, 10, None, None, 5, 1, None, None, None, None), 1),  # null pair list
         ("gm_coupling_pairs", (1, 20, 1, 10, 1, 1, -1, 1, None, None, None, None), 1),      # negative count
+        ("gm_coupling_rows", (1, 20, 1, 10, None, 3, 1, None, None, None, None), 1),   # null item list
+        ("gm_coupling_rows", (1, 20, 1, 10, 1, -1, 1, None, None, None, None), 1),     # negative count
+        ("gm_coupling_rows", (1, 21, 1, 10, 1, 1, 1, None, None, None, None), 1),      # window beyond 2 lmax
+        ("gm_coupling_rows", (1, 20, 1, 10, 1, 1, None, None, None, None, None), 1),   # no output
+        ("gm_lu_factor", (None, 10, 1, None), 1),                           # null matrix
+        ("gm_lu_factor", (1, 0, 1, None), 1),                               # n = 0
+        ("gm_lu_solve", (1, None, 10, 1, 1, None), 1),                      # null transpose
+        ("gm_lu_solve", (1, 1, 10, 1, 0, None), 1),                         # no right-hand side
         ("gm_toeplitz", (1, 10, 10, 3, 2, 2, None), 1),                     # l_toeplitz = lmax
         ("gm_toeplitz", (1, 10, 6, 7, 2, 2, None), 1),                      # l_exact > l_toeplitz
         ("gm_toeplitz", (1, 10, 6, 3, 2, 1, None), 1),                      # out aliases M
