@@ -239,8 +239,8 @@ def sims(args):
 
 def coupling_forms(args):
     """Config 4: the mode coupling at the benchmark Nside by exact 3j, by the chosen form's
-    wide-band Toeplitz, by the committed surrogate's narrow Toeplitz alone and with its
-    learned residual.  Per mask (the benchmark mask and ``--fresh-masks`` random ones): the
+    wide-band Toeplitz (with its spot check and fallback, and without), by the committed
+    surrogate's narrow Toeplitz alone and with its learned residual.  Per mask (the benchmark mask and ``--fresh-masks`` random ones): the
     time of the three workspaces' coupling, the relative Frobenius error of each block
     against the exact one, and the bandpower errors of T+P fields decoupled with each
     coupling against the exact decoupling of the same coupled spectra."""
@@ -264,7 +264,9 @@ def coupling_forms(args):
                                                                           device=dev))]
     for i in range(args.fresh_masks):
         masks.append((f"fresh random mask {i}", learned.random_mask(nside, rng, dev)))
+    wide = workspaces.nmt_params.fast_toeplitz
     variants = [("exact", "fp64", {}), ("toeplitz_wide", "fast", {}),
+                ("toeplitz_wide_unchecked", "fp64", {"l_toeplitz": wide[0], "l_exact": wide[1], "dl_band": wide[2]}),
                 ("toeplitz_narrow", "fp64", {"l_toeplitz": narrow[0], "l_exact": narrow[1], "dl_band": narrow[2]}),
                 ("learned", "learned", {})]
     out = []
