@@ -86,33 +86,44 @@ def _toeplitz_pairs(L, l_toeplitz, l_exact, dl_band, device):
     return lo.to(torch.int32), hi.to(torch.int32)
 
 
+_ITEMS_CACHE = {}
+
+
 def _toeplitz_items(L, l_toeplitz, l_exact, dl_band, device):
     """``gm_coupling_rows`` items ``(n, 3)`` int32 covering exactly the pairs of
     ``_toeplitz_pairs``: row segments (rows lo <= l_exact, the band rows, row l_toeplitz past
     its band) cut into 64-column waves, and one single-pair item per entry of column
-    l_toeplitz above the band."""
+    l_toeplitz above the band.  Built on the host once per geometry and kept on the device
+    (the 00, 02 and 22 workspaces of a mask share them, and a rebuild per call would be a
+    host round trip while the device waits)."""
     lt, le, dl = int(l_toeplitz), int(l_exact), int(dl_band)
-    i64 = dict(dtype=torch.int64, device=device)
-    ra = torch.arange(0, min(le, L - 1) + 1, **i64)
-    rb = torch.arange(le + 1, L, **i64)
-    rows = [ra, rb]
-    starts = [ra, rb]
-    counts = [L - ra, torch.clamp(L - rb, max=dl + 1)]
+    key = (int(L), lt, le, dl, str(device))
+    hit = _ITEMS_CACHE.get(key)
+    if hit is not None:
+        return hit
+    ra = np.arange(0, min(le, L - 1) + 1)
+    rb = np.arange(le + 1, L)
+    rows, starts, counts = [ra, rb], [ra, rb], [L - ra, np.minimum(L - rb, dl + 1)]
     if lt > le and lt + dl + 1 < L:
-        rows.append(torch.tensor([lt], **i64))
-        starts.append(torch.tensor([lt + dl + 1], **i64))
-        counts.append(torch.tensor([L - (lt + dl + 1)], **i64))
-    c1 = torch.arange(le + 1, max(le + 1, lt - dl), **i64)  # (lo, lt) above the band
+        rows.append(np.array([lt]))
+        starts.append(np.array([lt + dl + 1]))
+        counts.append(np.array([L - (lt + dl + 1)]))
+    c1 = np.arange(le + 1, max(le + 1, lt - dl))  # (lo, lt) above the band
     rows.append(c1)
-    starts.append(torch.full_like(c1, lt))
-    counts.append(torch.ones_like(c1))
-    row, start, count = torch.cat(rows), torch.cat(starts), torch.cat(counts)
+    starts.append(np.full_like(c1, lt))
+    counts.append(np.ones_like(c1))
+    row, start, count = np.concatenate(rows), np.concatenate(starts), np.concatenate(counts)
     keep = count > 0
     row, start, count = row[keep], start[keep], count[keep]
     nch = (count + 63) // 64
-    seg, k = _runs(torch.zeros_like(nch), nch, device)  # chunk k of segment seg
-    items = torch.stack([row[seg], start[seg] + 64 * k, torch.clamp(count[seg] - 64 * k, max=64)], 1)
-    return items.to(torch.int32).contiguous()
+    seg = np.repeat(np.arange(len(nch)), nch)
+    k = np.arange(len(seg)) - np.repeat(np.cumsum(nch) - nch, nch)  # chunk of its segment
+    items = np.stack([row[seg], start[seg] + 64 * k, np.minimum(count[seg] - 64 * k, 64)], 1).astype(np.int32)
+    out = torch.as_tensor(np.ascontiguousarray(items), device=device)
+    if len(_ITEMS_CACHE) >= 8:
+        _ITEMS_CACHE.clear()
+    _ITEMS_CACHE[key] = out
+    return out
 
 
 def coupling_blocks(window, lmax, toeplitz=None):
