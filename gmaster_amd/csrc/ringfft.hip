@@ -555,6 +555,33 @@ __device__ inline double2 fold_sum(const double2* F, int64_t col0, int mlo, int 
     return cmul(acc, phase_pi(m0, g.phi0_num, g.phi0_den, 1.0));
 }
 
+// fold_sum with the phase of residue k supplied: phk = e^{i pi k num / n} for 0 <= k < n, so
+// that a caller stepping k by a fixed stride advances it by one rotation instead of a
+// sincospi per k; the first alias m0 = k + j0 n carries phk (-1)^(j0 num).
+__device__ inline double2 fold_sum_ph(const double2* F, int64_t col0, int mlo, int mhi, int k, int n, int num,
+                                      double2 phk) {
+    int64_t t = ((int64_t)k - mlo) % n;
+    if (t < 0) t += n;
+    const int64_t m0 = mlo + t;
+    double2 acc = make_double2(0.0, 0.0);
+    bool neg = false;
+    for (int64_t m = m0; m <= mhi; m += n) {
+        const double2 v = F[m + col0];
+        acc = neg ? csub(acc, v) : cadd(acc, v);
+        if (num) neg = !neg;
+    }
+    const int64_t j0 = (m0 - k) / n;  // exact: m0 = k (mod n)
+    if (num && (j0 & 1)) phk = make_double2(-phk.x, -phk.y);
+    return cmul(acc, phk);
+}
+
+// e^{i pi (n - k) num / n} = (-1)^num conj(e^{i pi k num / n}) for 1 <= k < n; 1 for k = 0
+__device__ inline double2 mirror_phase(double2 phk, int k, int num) {
+    if (k == 0) return make_double2(1.0, 0.0);
+    const double2 c = make_double2(phk.x, -phk.y);
+    return num ? make_double2(-c.x, -c.y) : c;
+}
+
 template <bool BS>
 __global__ void k_ring_inv_s0(const double2* __restrict__ spec, int64_t ld, int64_t spec_map_stride,
                               int nside, int L, int qlo, int qhi, double* __restrict__ maps,
@@ -565,11 +592,15 @@ __global__ void k_ring_inv_s0(const double2* __restrict__ spec, int64_t ld, int6
     const Ring g = ring_info(nside, r);
     const int n = g.nphi, N = n / 2;
     const double2* F = spec + blockIdx.y * spec_map_stride + (int64_t)r * ld;
+    // e^{i pi k num / n}, advanced by the rotation of the thread stride
+    double2 phk = phase_pi(threadIdx.x, g.phi0_num, g.phi0_den, 1.0);
+    const double2 rot = phase_pi(blockDim.x, g.phi0_num, g.phi0_den, 1.0);
     for (int k = threadIdx.x; k <= N; k += blockDim.x) {
         // X[k] = sum_{m=k} F e^{i m phi0} + conj(sum_{m=-k, m>=1} F e^{i m phi0})
-        const double2 a = fold_sum(F, 0, 0, L - 1, k, n, g);
+        const double2 a = fold_sum_ph(F, 0, 0, L - 1, k, n, g.phi0_num, phk);
         const int kneg = (n - k) % n;
-        const double2 b = fold_sum(F, 0, 1, L - 1, kneg, n, g);
+        const double2 b = fold_sum_ph(F, 0, 1, L - 1, kneg, n, g.phi0_num, mirror_phase(phk, k, g.phi0_num));
+        phk = cmul(phk, rot);
         const double2 X = make_double2(a.x + b.x, a.y - b.y);
         if (k < N) buf[k] = X;
         else xn = X;
@@ -597,10 +628,14 @@ __global__ void k_ring_inv_s(const double2* __restrict__ spec, int64_t ld, int n
     const int64_t col0 = L - 1;  // F[m] lives at column m + L - 1
     // XQ = (Y(k) + conj Y(-k)) / 2 and XU = -(i/2)(Y(k) - conj Y(-k)); the folds are cheap
     // next to the FFT, so each component refolds instead of holding the other in registers.
+    const double2 rot = phase_pi(blockDim.x, g.phi0_num, g.phi0_den, 1.0);
     for (int comp = 0; comp < 2; ++comp) {
+        double2 phk = phase_pi(threadIdx.x, g.phi0_num, g.phi0_den, 1.0);
         for (int k = threadIdx.x; k <= N; k += blockDim.x) {
-            const double2 yp = fold_sum(F, col0, -(L - 1), L - 1, k, n, g);
-            const double2 ym = fold_sum(F, col0, -(L - 1), L - 1, (n - k) % n, n, g);
+            const double2 yp = fold_sum_ph(F, col0, -(L - 1), L - 1, k, n, g.phi0_num, phk);
+            const double2 ym = fold_sum_ph(F, col0, -(L - 1), L - 1, (n - k) % n, n, g.phi0_num,
+                                           mirror_phase(phk, k, g.phi0_num));
+            phk = cmul(phk, rot);
             double2 X;
             if (comp == 0) {
                 X = make_double2(0.5 * (yp.x + ym.x), 0.5 * (yp.y - ym.y));
