@@ -16,7 +16,7 @@ import re
 import sys
 from collections import defaultdict
 
-_KERNELS = ("k_coupling_pair_values", "k_toeplitz_correct", "k_bat_ana_h", "k_bat_ana", "k_bat_syn64", "k_bat_syn", "k_bat_finish", "k_bat_prep", "k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
+_KERNELS = ("k_coupling_pair_values", "k_coupling_rows", "k_toeplitz_correct", "k_bat_ana_h", "k_bat_ana", "k_bat_syn64", "k_bat_syn", "k_bat_finish", "k_bat_prep", "k_ana_s0", "k_ana_s", "k_syn_s0_mx", "k_syn_s_mx", "k_syn_s0", "k_syn_s", "k_coupling_pairs",
             "k_coupling", "k_toeplitz", "k_ring_fwd_s0", "k_ring_fwd_s", "k_ring_inv_s0", "k_ring_inv_s", "k_fold_s0", "k_fold_s",
             "k_finish_s0", "k_finish_s", "k_prep_s0", "k_prep_s", "k_lat_tables", "k_cl_partial", "k_cl_final",
             "k_bin_block", "k_bin_cell", "k_gauss_legendre", "k_wigner_d_table", "k_wigner_d_corr",
