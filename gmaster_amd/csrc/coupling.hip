@@ -85,7 +85,9 @@ __global__ __launch_bounds__(256) void k_coupling_rows(const double* __restrict_
     if (item >= nitems) return;
     const int l1 = __builtin_amdgcn_readfirstlane(items[3 * item]);
     const int l2_0 = __builtin_amdgcn_readfirstlane(items[3 * item + 1]);
-    const int nlive = __builtin_amdgcn_readfirstlane(items[3 * item + 2]);
+    int nlive = __builtin_amdgcn_readfirstlane(items[3 * item + 2]);
+    if (l1 < 0 || l2_0 < l1 || l2_0 >= L || nlive < 1) return;  // malformed item: nothing to do
+    nlive = min(nlive, min(64, L - l2_0));
     LdsWin win{win_buf[wave], Wp, lmax_w, l2_0 - l1, lane};
     double s[4];
     couple_wave_lane(win, lmax_w, g, L, l1, l2_0, lane, s);
@@ -93,13 +95,18 @@ __global__ __launch_bounds__(256) void k_coupling_rows(const double* __restrict_
 }
 
 // The four row sums (without the column factor) of each listed pair, vals[k][0..3].
+// (A malformed pair, outside 0 <= lo <= hi < L, gets NaN.)
 __global__ void k_coupling_pair_values(const double* __restrict__ Wp, int lmax_w, const double* __restrict__ g,
-                                       const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
+                                       int L, const int32_t* __restrict__ lo, const int32_t* __restrict__ hi,
                                        int64_t npairs, double* __restrict__ vals) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npairs) return;
+    const int l1 = lo[k], l2 = hi[k];
     double s[4];
-    couple_pair(Wp, lmax_w, g, lo[k], hi[k], s);
+    if (l1 < 0 || l2 < l1 || l2 >= L)
+        s[0] = s[1] = s[2] = s[3] = __builtin_nan("");
+    else
+        couple_pair(Wp, lmax_w, g, l1, l2, s);
     for (int q = 0; q < 4; ++q) vals[k * 4 + q] = s[q];
 }
 
@@ -111,6 +118,7 @@ __global__ void k_coupling_pairs(const double* __restrict__ Wp, int lmax_w, cons
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npairs) return;
     const int l1 = lo[k], l2 = hi[k];
+    if (l1 < 0 || l2 < l1 || l2 >= L) return;  // malformed pair: nothing to do
     double s[4];
     couple_pair(Wp, lmax_w, g, l1, l2, s);
     store_pair(l1, l2, L, s, m00, m02, m22p, m22m);
@@ -248,7 +256,7 @@ extern "C" int gm_coupling_pair_values(const double* window, int64_t lmax_window
     if (npairs < 0) return fail(GM_E_ARG, "gm_coupling_pair_values: negative pair count");
     if (npairs == 0) return GM_OK;
     hipLaunchKernelGGL(k_coupling_pair_values, dim3((unsigned)((npairs + 127) / 128)), dim3(128), 0,
-                       (hipStream_t)stream, window, (int)lmax_window, g, lo, hi, npairs, vals);
+                       (hipStream_t)stream, window, (int)lmax_window, g, (int)lmax + 1, lo, hi, npairs, vals);
     return check_launch("gm_coupling_pair_values");
 }
 

@@ -90,12 +90,15 @@ _ITEMS_CACHE = {}
 
 
 def _toeplitz_items(L, l_toeplitz, l_exact, dl_band, device):
-    """``gm_coupling_rows`` items ``(n, 3)`` int32 covering exactly the pairs of
-    ``_toeplitz_pairs``: row segments (rows lo <= l_exact, the band rows, row l_toeplitz past
-    its band) cut into 64-column waves, and one single-pair item per entry of column
-    l_toeplitz above the band.  Built on the host once per geometry and kept on the device
-    (the 00, 02 and 22 workspaces of a mask share them, and a rebuild per call would be a
-    host round trip while the device waits)."""
+    """``(items, lo, hi)`` covering exactly the pairs of ``_toeplitz_pairs``: ``items``
+    ``(n, 3)`` int32 for ``gm_coupling_rows`` -- row segments (rows lo <= l_exact, the band
+    rows, row l_toeplitz past its band) cut into 64-column waves -- and int32 pair lists for
+    ``gm_coupling_pairs``: the entries of column l_toeplitz above the band and, when a band
+    row's length leaves at most 8 entries past its last whole wave, those entries (one
+    diagonal each, consecutive rows sharing a wave there instead of a wave holding 1-8
+    lanes).  Built on the host once per geometry and kept on the device (the 00, 02 and 22
+    workspaces of a mask share them, and a rebuild per call would be a host round trip
+    while the device waits)."""
     lt, le, dl = int(l_toeplitz), int(l_exact), int(dl_band)
     key = (int(L), lt, le, dl, str(device))
     hit = _ITEMS_CACHE.get(key)
@@ -103,15 +106,14 @@ def _toeplitz_items(L, l_toeplitz, l_exact, dl_band, device):
         return hit
     ra = np.arange(0, min(le, L - 1) + 1)
     rb = np.arange(le + 1, L)
-    rows, starts, counts = [ra, rb], [ra, rb], [L - ra, np.minimum(L - rb, dl + 1)]
+    full = 64 * ((dl + 1) // 64)
+    split = full > 0 and (dl + 1) - full <= 8
+    bw = full if split else dl + 1
+    rows, starts, counts = [ra, rb], [ra, rb], [L - ra, np.minimum(L - rb, bw)]
     if lt > le and lt + dl + 1 < L:
         rows.append(np.array([lt]))
         starts.append(np.array([lt + dl + 1]))
         counts.append(np.array([L - (lt + dl + 1)]))
-    c1 = np.arange(le + 1, max(le + 1, lt - dl))  # (lo, lt) above the band
-    rows.append(c1)
-    starts.append(np.full_like(c1, lt))
-    counts.append(np.ones_like(c1))
     row, start, count = np.concatenate(rows), np.concatenate(starts), np.concatenate(counts)
     keep = count > 0
     row, start, count = row[keep], start[keep], count[keep]
@@ -119,7 +121,16 @@ def _toeplitz_items(L, l_toeplitz, l_exact, dl_band, device):
     seg = np.repeat(np.arange(len(nch)), nch)
     k = np.arange(len(seg)) - np.repeat(np.cumsum(nch) - nch, nch)  # chunk of its segment
     items = np.stack([row[seg], start[seg] + 64 * k, np.minimum(count[seg] - 64 * k, 64)], 1).astype(np.int32)
-    out = torch.as_tensor(np.ascontiguousarray(items), device=device)
+    c1 = np.arange(le + 1, max(le + 1, lt - dl))  # (lo, lt) above the band
+    plo, phi = [c1], [np.full_like(c1, lt)]
+    if split:
+        for t in range(full, dl + 1):  # band entries past the whole waves, one diagonal each
+            r = rb[rb + t < L]
+            plo.append(r)
+            phi.append(r + t)
+    lo, hi = np.concatenate(plo), np.concatenate(phi)
+    out = (torch.as_tensor(np.ascontiguousarray(items), device=device),
+           torch.as_tensor(lo.astype(np.int32), device=device), torch.as_tensor(hi.astype(np.int32), device=device))
     if len(_ITEMS_CACHE) >= 8:
         _ITEMS_CACHE.clear()
     _ITEMS_CACHE[key] = out
@@ -154,10 +165,12 @@ def coupling_blocks(window, lmax, toeplitz=None):
                   stream)
     else:
         lt, le, dl = tkey
-        items = _toeplitz_items(L, lt, le, dl, dev)
+        items, plo, phi = _toeplitz_items(L, lt, le, dl, dev)
         exact = {k: torch.zeros((L, L), dtype=torch.float64, device=dev) for k in names}
         _lib.call("gm_coupling_rows", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(items),
                   items.shape[0], *(_lib.ptr(exact[k]) for k in names), stream)
+        _lib.call("gm_coupling_pairs", _lib.ptr(wp), len(w) - 1, _lib.ptr(g), lmax, _lib.ptr(plo), _lib.ptr(phi),
+                  plo.numel(), *(_lib.ptr(exact[k]) for k in names), stream)
         blocks = {}
         for k in names:
             blocks[k] = torch.empty((L, L), dtype=torch.float64, device=dev)

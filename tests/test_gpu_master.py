@@ -187,7 +187,7 @@ def test_coupling_rows_match_full_kernel(lmax, lw):
     full = coupling_blocks(w, lmax)
     wp = (w * (2 * torch.arange(len(w), dtype=torch.float64, device="cuda") + 1) / (4 * np.pi)).contiguous()
     lt, le, dl = lmax // 2, lmax // 10, lmax // 8
-    items = _toeplitz_items(L, lt, le, dl, "cuda")
+    items, _, _ = _toeplitz_items(L, lt, le, dl, "cuda")
     print(f"coupling rows lmax={lmax} lw={lw}: {items.shape[0]} items", flush=True)
     out = {k: torch.full((L, L), float("nan"), dtype=torch.float64, device="cuda") for k in ("00", "02", "22p", "22m")}
     _lib.call("gm_coupling_rows", _lib.ptr(wp), len(w) - 1, _lib.ptr(_g_table(L + 1, "cuda")), lmax, _lib.ptr(items),

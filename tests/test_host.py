@@ -77,19 +77,23 @@ def test_toeplitz_pair_list_covers_the_exact_entries(lt, le, dl):
 
 @pytest.mark.parametrize("L", [41, 200])
 @pytest.mark.parametrize("lt,le,dl", [(6, 3, 2), (16, 4, 3), (30, 30, 0), (20, 5, 30), (39, 1, 0), (25, 10, 20),
-                                      (150, 64, 70)])
+                                      (150, 64, 70), (150, 20, 64), (120, 10, 128)])
 def test_toeplitz_row_items_cover_the_exact_entries(L, lt, le, dl):
     """The wave items of gm_coupling_rows (row l1, columns l2_0 .. l2_0 + nlive - 1, at most
-    64 per item, inside the triangle) cover each exact entry of the fill exactly once."""
+    64 per item, inside the triangle) and the single pairs beside them cover each exact entry
+    of the fill exactly once (band widths that leave a short remainder split it off)."""
     from gmaster_amd.workspaces import _toeplitz_items
     from oracle.toeplitz import exact_pairs
 
     if lt >= L - 1:
         pytest.skip("l_toeplitz must be below lmax")
-    it = _toeplitz_items(L, lt, le, dl, "cpu").numpy().astype(np.int64)
+    items, plo, phi = _toeplitz_items(L, lt, le, dl, "cpu")
+    it = items.numpy().astype(np.int64)
     l1, l20, n = it[:, 0], it[:, 1], it[:, 2]
     assert np.all((n >= 1) & (n <= 64)) and np.all(l1 <= l20) and np.all(l20 + n <= L)
-    got = np.concatenate([a * L + b + np.arange(k) for a, b, k in it])
+    plo, phi = plo.numpy().astype(np.int64), phi.numpy().astype(np.int64)
+    assert np.all(plo <= phi) and np.all(phi < L)
+    got = np.concatenate([a * L + b + np.arange(k) for a, b, k in it] + [plo * L + phi])
     r, c = exact_pairs(L, lt, le, dl)
     want = np.unique(np.minimum(r, c) * L + np.maximum(r, c))
     assert len(np.unique(got)) == len(got)
