@@ -68,6 +68,9 @@ template <> struct Mfma16<double> {
 #ifndef GM_BAT_ANA_R
 #define GM_BAT_ANA_R 1    // ring sets per lane in the FP32 analysis
 #endif
+#ifndef GM_BAT_SUPER_H
+#define GM_BAT_SUPER_H 4  // 32-degree tiles per cross-wave reduction in the FP16 analysis
+#endif
 #ifndef GM_BAT_SYN_NW
 #define GM_BAT_SYN_NW 4
 #endif
@@ -80,6 +83,9 @@ template <> struct Mfma16<double> {
 
 constexpr int kTile = 32;       // degrees per analysis tile (16 of each parity)
 constexpr int kSuper = 2;       // analysis tiles per cross-wave reduction
+// The same for the FP16 analysis (fewer cross-wave reductions and barriers; 2.5% faster at
+// Nside 4096); spin-2-only batches keep 2, since they spill registers at 4.
+template <int K0> constexpr int super_h() { return K0 > 0 ? GM_BAT_SUPER_H : 2; }
 constexpr int kTabExtra = 2;    // table degrees beyond lmax + 1 (tables of lmax + 2)
 constexpr int kAlphaTail = 128;  // zero doubles the caller appends to the alpha table
 constexpr int kCoefTail = 3 * kTile;  // spare degrees (x the channel stride) after the coefficients
@@ -293,18 +299,24 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
         // acc[h][p][v] at lane (kq, n): channel Mfma16<F>::row(kq, v), degree ls + 32 h + 2 n + p.
         // Each wave parks its super-tile in its own (now free) transpose buffer in output
         // order [degree][slot], then the waves sum slot by slot (consecutive threads read
-        // consecutive words).
+        // consecutive words).  FP32 rows 4 kq .. 4 kq + 3 are consecutive slots: one 16-byte
+        // store per accumulator (SLOT is a multiple of 4, so it stays aligned).
         constexpr int NOUT = kSuper * kTile * SLOT;
         static_assert(NOUT <= 2 * 16 * kRow, "super-tile must fit the transpose buffer");
 #pragma unroll
         for (int h = 0; h < kSuper; ++h)
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
+            for (int p = 0; p < 2; ++p) {
+                if constexpr (sizeof(F) == 4) {
+                    if (4 * kq < SLOT) *reinterpret_cast<acc_t*>(mytb + (h * kTile + 2 * ich + p) * SLOT + 4 * kq) = acc[h][p];
+                } else {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int c = Mfma16<F>::row(kq, v);
-                    if (c < SLOT) mytb[(h * kTile + 2 * ich + p) * SLOT + c] = acc[h][p][v];
+                    for (int v = 0; v < 4; ++v) {
+                        const int c = Mfma16<F>::row(kq, v);
+                        if (c < SLOT) mytb[(h * kTile + 2 * ich + p) * SLOT + c] = acc[h][p][v];
+                    }
                 }
+            }
         __syncthreads();
         for (int u = threadIdx.x; u < NOUT; u += T) {
             const int l = ls + u / SLOT;
@@ -346,14 +358,14 @@ __device__ inline h2 split16(float v) {
     return h2{(_Float16)hf, (_Float16)(v - hf)};
 }
 
-// The same split as one packed word (hi in bits 0-15, lo in 16-31) from one v_cvt_pk_f16_f32
-// (round to nearest even); written out so that the two halves, stored apart, still come
-// from a single convert.
+// v = hi + lo as one packed word (hi in bits 0-15, lo in 16-31): hi = v rounded to FP16
+// (v_cvt_f16_f32), lo = (v - hi) rounded to FP16 by v_fma_mixhi_f16 (FP16 x FP32 + FP32,
+// one rounding, into the high half; v - hi is exact in FP32, so hi + lo carries v to
+// ~2^-22 relative).  Two ALU operations; written out because the compiler expands the same
+// C++ into an integer conversion sequence several times longer.
 __device__ inline unsigned split16_packed(float v) {
-    const float hf = __uint_as_float(__float_as_uint(v) & 0xFFFFE000u);
-    const float r = v - hf;
     unsigned out;
-    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(out) : "v"(hf), "v"(r));
+    asm("v_cvt_f16_f32 %0, %1\n\tv_fma_mixhi_f16 %0, %0, %2, %1 op_sel_hi:[1,0,0]" : "=&v"(out) : "v"(v), "v"(-1.0f));
     return out;
 }
 
@@ -365,6 +377,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
                                                          const int* __restrict__ nact, int jbeg, int jend,
                                                          float* __restrict__ partial, int64_t tsize) {
     typedef Chan<K0, K2> C;
+    constexpr int kSuperH = super_h<K0>();
     constexpr int T = 64 * NW;
     constexpr int SLOT = C::ana_stride;
     constexpr int TILE = 2 * 16 * kRowH;  // words per wave: parity x 16 degree rows x row
@@ -427,10 +440,10 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
     const double* al_ = alpha + table_base(m, Lt) - m;
     float* part = partial + ((int64_t)blockIdx.x * tsize + table_base(m, Lt) - m) * SLOT;
     unsigned* mytw = tb[w];
-    for (int ls = m; ls <= L; ls += kSuper * kTile) {
-        f4 acc[kSuper][2];
+    for (int ls = m; ls <= L; ls += kSuperH * kTile) {
+        f4 acc[kSuperH][2];
 #pragma unroll
-        for (int h = 0; h < kSuper; ++h) {
+        for (int h = 0; h < kSuperH; ++h) {
             const int l0 = ls + h * kTile;
             double a[kTile];
 #pragma unroll
@@ -480,19 +493,23 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
             wave_sync();
         }
         // acc[h][p][v] at lane (kg, n): channel 4 kg + v, degree ls + 32 h + 2 n + p; unscaled
-        // into the wave's (now free) buffer in output order, then summed over the waves
-        constexpr int NOUT = kSuper * kTile * SLOT;
+        // into the wave's (now free) buffer in output order (one 16-byte store of 4
+        // consecutive slots per accumulator), then summed over the waves
+        constexpr int NOUT = kSuperH * kTile * SLOT;
         static_assert(NOUT <= TILE, "super-tile must fit the buffer");
+        static_assert(SLOT % 4 == 0, "16-byte stores of 4 slots");
         float* mytb = reinterpret_cast<float*>(tb[w]);
+        if (4 * kg < SLOT) {
 #pragma unroll
-        for (int h = 0; h < kSuper; ++h)
+            for (int h = 0; h < kSuperH; ++h)
 #pragma unroll
-            for (int p = 0; p < 2; ++p)
+                for (int p = 0; p < 2; ++p) {
+                    f4 o;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int c = 4 * kg + v;
-                    if (c < SLOT) mytb[(h * kTile + 2 * ich + p) * SLOT + c] = acc[h][p][v] * inv[v];
+                    for (int v = 0; v < 4; ++v) o[v] = acc[h][p][v] * inv[v];
+                    *reinterpret_cast<f4*>(mytb + (h * kTile + 2 * ich + p) * SLOT + 4 * kg) = o;
                 }
+        }
         __syncthreads();
         for (int u = threadIdx.x; u < NOUT; u += T) {
             const int l = ls + u / SLOT;
