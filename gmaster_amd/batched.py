@@ -27,6 +27,17 @@ _PLANS = {}
 # Ring pairs with sin^2(theta) below this contract in FP64 (the 1/sin^2 channel's weight
 # stays <= 16 in FP32).
 POLAR_SIN2 = 1.0 / 16.0
+# The FP16 hi/lo analysis (every pass of a refinement: the next pass removes a pass's
+# rounding, and the last pass's rounding scales with its residual) keeps FP64 only nearer
+# the poles: its split is the largest multiple of its workgroup's ring pairs (ANA_BLOCK)
+# between the sin^2(theta) = 1/24 and 1/96 splits, or the 1/48 split where no multiple
+# falls in that range.  Whole workgroups matter: a last workgroup with a few live waves
+# costs about as much as a full one for every order (Nside 4096, 8192 ring pairs: split
+# 7680 = 15 workgroups; analysis pass 143.7 ms against 156.8 ms at the 1/16 split 6928,
+# 175.8 ms at 7749; bandpowers unchanged, alm errors vs FP64 within 1e-6).
+RES_SIN2_RANGE = (1.0 / 24.0, 1.0 / 96.0)
+RES_SIN2_FALLBACK = 1.0 / 48.0
+ANA_BLOCK = 512  # ring pairs per FP16/FP32 analysis workgroup (8 waves of 64)
 # With refinement passes (n_iter >= 1) the first analysis contracts on FP16 hi/lo products
 # too: the refinement subtracts A S (alm_0 - a) from the first alms, which removes their
 # rounding with the rest of the quadrature error (measured at Nside 4096: alm errors vs
@@ -56,6 +67,7 @@ class BatPlan:
     nb64: int
     tsize: int
     tsize64: int           # table entries of the orders m < mpol
+    res: tuple             # (jsplit, mpol, nb32, nb64, tsize64) of the FP16 hi/lo analysis
 
 
 def _nactive_pairs(nside, lmax, polar_skip):
@@ -74,10 +86,27 @@ def _nactive_pairs(nside, lmax, polar_skip):
     return np.searchsorted(-best, -np.arange(L), side="right").astype(np.int32)
 
 
+def _split_at(nside, sin2):
+    _, sin = sht._ring_slots(nside, 0)
+    return int(np.count_nonzero(sin * sin >= sin2))
+
+
 def polar_split(nside):
     """Number of ring pairs (from the equator) with sin^2(theta) >= POLAR_SIN2."""
-    _, sin = sht._ring_slots(nside, 0)
-    return int(np.count_nonzero(sin * sin >= POLAR_SIN2))
+    return _split_at(nside, POLAR_SIN2)
+
+
+def residual_split(nside):
+    """First ring pair of the FP16 hi/lo analysis contracted in FP64 (RES_SIN2_RANGE)."""
+    lo, hi = _split_at(nside, RES_SIN2_RANGE[0]), _split_at(nside, RES_SIN2_RANGE[1])
+    j = hi // ANA_BLOCK * ANA_BLOCK
+    return j if j > 0 and j >= lo else _split_at(nside, RES_SIN2_FALLBACK)
+
+
+def _split_fields(nside, nact, Lt, jsplit):
+    mpol = int(np.count_nonzero(nact > jsplit))
+    return (jsplit, mpol, int(_lib.query("gm_bat_blocks", nside, jsplit, 0)),
+            int(_lib.query("gm_bat_blocks", nside, jsplit, 1)), mpol * (Lt + 8) - mpol * (mpol - 1) // 2)
 
 
 def plan(nside, lmax, device=None, polar_skip=True):
@@ -90,19 +119,19 @@ def plan(nside, lmax, device=None, polar_skip=True):
         return bp
     nside, lmax = int(nside), int(lmax)
     tab = sht.plan(nside, lmax + 2, 0, device, polar_skip)
-    jsplit = polar_split(nside)
+    if nside >= ANA_BLOCK and int(_lib.query("gm_bat_blocks", nside, ANA_BLOCK + 1, 0)) != 2:
+        raise RuntimeError("ANA_BLOCK does not match the library's analysis workgroup")
     nact = _nactive_pairs(nside, lmax, polar_skip)
-    mpol = int(np.count_nonzero(nact > jsplit))
     Lt = lmax + 3  # table degrees
+    jsplit, mpol, nb32, nb64, tsize64 = _split_fields(nside, nact, Lt, polar_split(nside))
     alpha = torch.zeros(tab.alpha.numel() + ALPHA_TAIL, dtype=torch.float64, device=device)
     alpha[: tab.alpha.numel()].copy_(tab.alpha)
     bp = BatPlan(
         nside=nside, lmax=lmax, device=device, tab=tab, alpha=alpha,
         nact=torch.as_tensor(nact, dtype=torch.int32, device=device),
-        jsplit=jsplit, mpol=mpol, tsize64=mpol * (Lt + 8) - mpol * (mpol - 1) // 2,
-        nb32=int(_lib.query("gm_bat_blocks", nside, jsplit, 0)),
-        nb64=int(_lib.query("gm_bat_blocks", nside, jsplit, 1)),
+        jsplit=jsplit, mpol=mpol, tsize64=tsize64, nb32=nb32, nb64=nb64,
         tsize=int(_lib.query("gm_table_size", lmax + 2)),
+        res=_split_fields(nside, nact, Lt, residual_split(nside)),
     )
     _PLANS[key] = bp
     return bp
@@ -127,16 +156,16 @@ def analysis(bp, spec0, spec2, alm0, alm2, mode, residual=False):
     _check(k0, k2)
     st = _lib.stream_ptr(bp.device)
     S = int(_lib.query("gm_bat_stride", k0, k2, 0))
-    part32 = torch.empty(bp.nb32 * bp.tsize * S, dtype=torch.float32, device=bp.device) if bp.nb32 else None
-    part64 = (torch.empty(bp.nb64 * bp.tsize64 * S, dtype=torch.float64, device=bp.device)
-              if bp.nb64 and bp.mpol else None)
+    jsplit, mpol, nb32, nb64, tsize64 = bp.res if residual else (bp.jsplit, bp.mpol, bp.nb32, bp.nb64, bp.tsize64)
+    part32 = torch.empty(nb32 * bp.tsize * S, dtype=torch.float32, device=bp.device) if nb32 else None
+    part64 = torch.empty(nb64 * tsize64 * S, dtype=torch.float64, device=bp.device) if nb64 and mpol else None
     t = bp.tab
     _lib.call("gm_bat_ana_res" if residual else "gm_bat_ana", _lib.ptr(spec0), k0, spec0.shape[2] if k0 else 0, spec0.stride(0) if k0 else 0,
               _lib.ptr(spec2), k2, spec2.shape[2] if k2 else 0, spec2.stride(0) if k2 else 0, bp.nside, bp.lmax,
-              _lib.ptr(bp.alpha), _lib.ptr(t.smant), _lib.ptr(t.sexp), _lib.ptr(bp.nact), bp.jsplit, bp.mpol,
+              _lib.ptr(bp.alpha), _lib.ptr(t.smant), _lib.ptr(t.sexp), _lib.ptr(bp.nact), jsplit, mpol,
               _lib.ptr(part32), _lib.ptr(part64), st)
     _lib.call("gm_bat_finish", _lib.ptr(part32), _lib.ptr(part64), k0, k2, bp.nside, bp.lmax, _lib.ptr(bp.nact),
-              bp.jsplit, bp.mpol, _lib.ptr(t.P), _lib.ptr(alm0), alm0.stride(0) if k0 else 0, _lib.ptr(alm2),
+              jsplit, mpol, _lib.ptr(t.P), _lib.ptr(alm0), alm0.stride(0) if k0 else 0, _lib.ptr(alm2),
               alm2.stride(1) if k2 else 0, int(mode), st)
 
 
