@@ -594,7 +594,8 @@ def main():
                        "hi/lo ring blocks on the matrix unit + FP64 polar blocks)", 2 * (2 + 12) * T0,
                        28.0 / (4.0 / PEAK_FP64 + 3 * 24.0 / PEAK_F16), "fp64 valu + f16 mfma (3 hi/lo products)", st)
         e["pipe"] = ("FP64 VALU recurrence (78.6 TFLOP/s) + FP16 MFMA contraction (2.5 PFLOP/s, 3 products per "
-                     "FP32 product): op-mix peak; the FP16 split (4 VALU ops per mu) is not in the FLOP count")
+                     "FP32 product): op-mix peak; the FP16 split (3 VALU ops per mu: FP64 -> FP32 convert, FP16 "
+                     "convert, mixed fma) is not in the FLOP count")
         e["traffic"] = _measured_traffic(nside, "k_bat_ana_h")
         return e
 
@@ -652,8 +653,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("f64" if args.form == "fp64" else
-                  "f64 recurrence; f16 hi/lo matrix products in the analyses, f32 in the syntheses (f64 within "
-                  "14.5 deg of the poles); f64 coupling and LU"),
+                  "f64 recurrence; f16 hi/lo matrix products in the analyses (f64 within 5.9 deg of the poles), "
+                  "f32 in the syntheses (f64 within 14.5 deg of the poles); f64 coupling and LU"),
         "data": "synthetic",
         "form": args.form,
         "fp64_s": fp64_s,
