@@ -492,32 +492,33 @@ __global__ __launch_bounds__(64 * NW, 4) void k_bat_ana_h(const double2* __restr
                 }
             wave_sync();
         }
-        // acc[h][p][v] at lane (kg, n): channel 4 kg + v, degree ls + 32 h + 2 n + p; unscaled
-        // into the wave's (now free) buffer in output order (one 16-byte store of 4
-        // consecutive slots per accumulator), then summed over the waves
-        constexpr int NOUT = kSuperH * kTile * SLOT;
-        static_assert(NOUT <= TILE, "super-tile must fit the buffer");
+        // acc[h][p][v] at lane (kg, n): channel 4 kg + v, degree ls + 32 h + 2 n + p.  Each wave
+        // parks its accumulators, unscaled, in its (now free) buffer in lane order (region
+        // (h, p), lane, 4 values: consecutive lanes store consecutive 16 bytes); thread t of the
+        // workgroup then sums region t / 64, lane t % 64 over the waves (reads in the same
+        // order) and stores the 4 slots 4 kg .. 4 kg + 3 of its degree as one 16-byte write.
+        constexpr int NREG = kSuperH * 2 * 64;  // (region, lane) chunks of 4 values
+        static_assert(4 * NREG <= TILE, "super-tile must fit the buffer");
         static_assert(SLOT % 4 == 0, "16-byte stores of 4 slots");
         float* mytb = reinterpret_cast<float*>(tb[w]);
-        if (4 * kg < SLOT) {
 #pragma unroll
-            for (int h = 0; h < kSuperH; ++h)
+        for (int h = 0; h < kSuperH; ++h)
 #pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    f4 o;
+            for (int p = 0; p < 2; ++p) {
+                f4 o;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) o[v] = acc[h][p][v] * inv[v];
-                    *reinterpret_cast<f4*>(mytb + (h * kTile + 2 * ich + p) * SLOT + 4 * kg) = o;
-                }
-        }
+                for (int v = 0; v < 4; ++v) o[v] = acc[h][p][v] * inv[v];
+                *reinterpret_cast<f4*>(mytb + ((2 * h + p) * 64 + ln) * 4) = o;
+            }
         __syncthreads();
-        for (int u = threadIdx.x; u < NOUT; u += T) {
-            const int l = ls + u / SLOT;
-            if (l > L) continue;
-            float sum = 0.f;
+        for (int t = threadIdx.x; t < NREG; t += T) {
+            const int reg = t >> 6, lane = t & 63, n = lane & 15, g = lane >> 4;
+            const int l = ls + 32 * (reg >> 1) + 2 * n + (reg & 1);
+            if (4 * g >= SLOT || l > L) continue;
+            f4 sum = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int ww = 0; ww < NW; ++ww) sum += reinterpret_cast<const float*>(tb[ww])[u];
-            part[(int64_t)ls * SLOT + u] = sum;
+            for (int ww = 0; ww < NW; ++ww) sum += *reinterpret_cast<const f4*>(reinterpret_cast<const float*>(tb[ww]) + 4 * t);
+            *reinterpret_cast<f4*>(part + (int64_t)l * SLOT + 4 * g) = sum;
         }
         __syncthreads();  // the next super-tile overwrites the buffers
     }
