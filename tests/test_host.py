@@ -150,3 +150,29 @@ def test_coupling_precision_setting():
     nmt.set_coupling_precision("auto")
     with pytest.raises(KeyError):
         nmt.set_coupling_precision("fp16")
+
+
+@pytest.mark.parametrize("nside", [16, 64, 256, 1024, 2048, 4096, 8192])
+def test_residual_polar_split_is_whole_workgroups_or_the_fallback(nside):
+    """The FP16 analysis's FP64 polar split: a whole number of analysis workgroups between
+    the sin^2 = 1/24 and 1/96 splits when one exists there, else the 1/48 split; never
+    closer to the pole than 1/96 nor further than 1/24 (checked on the ring geometry)."""
+    from gmaster_amd import batched
+
+    z, sin = sht._ring_slots(nside, 0)
+    s2 = sin * sin
+    j = batched.residual_split(nside)
+    lo = int(np.count_nonzero(s2 >= 1.0 / 24.0))
+    hi = int(np.count_nonzero(s2 >= 1.0 / 96.0))
+    assert lo <= j <= hi
+    blocks = [k * batched.ANA_BLOCK for k in range(1, 2 * nside // batched.ANA_BLOCK + 1)]
+    inside = [b for b in blocks if lo <= b <= hi]
+    if inside:
+        assert j == max(inside) and j % batched.ANA_BLOCK == 0
+    else:
+        assert j == int(np.count_nonzero(s2 >= 1.0 / 48.0))
+    # ring pairs are ordered from the equator: every pair before the split is at least as far
+    # from the pole as every pair after it
+    assert np.all(np.diff(s2) <= 0)
+    if 0 < j < len(s2):
+        assert s2[j - 1] >= s2[j]
