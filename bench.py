@@ -534,7 +534,8 @@ def main():
         return {
             "kernel": "k_bat_ana (T + mask + Q/U analysis on one recurrence, gm_bat_ana: FP32 ring blocks on the "
                       "matrix unit + FP64 polar blocks)",
-            "bound": "fp64 valu + fp32 mfma",
+            "bound": "valu",
+            "ops": "fp64 valu + fp32 mfma",
             "pipe": "FP64 VALU recurrence (78.6 TFLOP/s) + FP32 MFMA contraction (157.3 TFLOP/s): op-mix peak",
             "achieved": achieved / 1e12,
             "peak": peak / 1e12,
@@ -580,11 +581,13 @@ def main():
         return {"kernel": kernel, "bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "bytes_per_launch": nbytes, "bytes": what, "mean_launch_ms": ms}
 
-    def flop_entry(name, kernel, flop, peak, pipe, st=None):
+    def flop_entry(name, kernel, flop, peak, ops, st=None):
+        # bound "valu": the FP64 recurrence on the vector ALU sets these kernels' time (matrix
+        # and vector instructions issue one after the other on a SIMD); "ops" names the mix
         ms = (st or stages).get(name, {}).get("mean_ms", float("nan"))
         ach = flop / (ms * 1e-3)
-        return {"kernel": kernel, "bound": pipe, "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                "frac": ach / peak, "flop_per_launch": flop, "mean_launch_ms": ms}
+        return {"kernel": kernel, "bound": "valu", "ops": ops, "achieved": ach / 1e12, "peak": peak / 1e12,
+                "unit": "TFLOP/s", "frac": ach / peak, "flop_per_launch": flop, "mean_launch_ms": ms}
 
     def roof_residual(st):
         # the analyses on FP16 hi/lo products (the largest share of the step): per (l, m, ring
