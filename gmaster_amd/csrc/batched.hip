@@ -297,34 +297,35 @@ __global__ __launch_bounds__(64 * NW, sizeof(F) == 4 ? 4 : 2) void k_bat_ana(con
             }
         }
         // acc[h][p][v] at lane (kq, n): channel Mfma16<F>::row(kq, v), degree ls + 32 h + 2 n + p.
-        // Each wave parks its super-tile in its own (now free) transpose buffer in output
-        // order [degree][slot], then the waves sum slot by slot (consecutive threads read
-        // consecutive words).  FP32 rows 4 kq .. 4 kq + 3 are consecutive slots: one 16-byte
-        // store per accumulator (SLOT is a multiple of 4, so it stays aligned).
-        constexpr int NOUT = kSuper * kTile * SLOT;
-        static_assert(NOUT <= 2 * 16 * kRow, "super-tile must fit the transpose buffer");
+        // Each wave parks its accumulators in its (now free) transpose buffer in lane order
+        // (region (h, p), lane, 4 values: consecutive lanes store consecutive chunks); thread t
+        // of the workgroup sums region t / 64, lane t % 64 over the waves (reads in the same
+        // order) and stores the 4 channels of its degree: FP32 rows 4 kq .. 4 kq + 3 are
+        // consecutive slots (one 16-byte write), FP64 rows kq, kq + 4, kq + 8, kq + 12 are not.
+        constexpr int NREG = kSuper * 2 * 64;  // (region, lane) chunks of 4 values
+        static_assert(4 * NREG <= 2 * 16 * kRow, "super-tile must fit the transpose buffer");
+        static_assert(SLOT % 4 == 0, "16-byte stores of 4 slots");
 #pragma unroll
         for (int h = 0; h < kSuper; ++h)
 #pragma unroll
-            for (int p = 0; p < 2; ++p) {
-                if constexpr (sizeof(F) == 4) {
-                    if (4 * kq < SLOT) *reinterpret_cast<acc_t*>(mytb + (h * kTile + 2 * ich + p) * SLOT + 4 * kq) = acc[h][p];
-                } else {
+            for (int p = 0; p < 2; ++p) *reinterpret_cast<acc_t*>(mytb + ((2 * h + p) * 64 + ln) * 4) = acc[h][p];
+        __syncthreads();
+        for (int t = threadIdx.x; t < NREG; t += T) {
+            const int reg = t >> 6, lane = t & 63, n = lane & 15, g = lane >> 4;
+            const int l = ls + 32 * (reg >> 1) + 2 * n + (reg & 1);
+            if (l > L) continue;
+            acc_t sum = {};
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int c = Mfma16<F>::row(kq, v);
-                        if (c < SLOT) mytb[(h * kTile + 2 * ich + p) * SLOT + c] = acc[h][p][v];
-                    }
+            for (int ww = 0; ww < NW; ++ww) sum += *reinterpret_cast<const acc_t*>(&tb[ww][4 * t]);
+            if constexpr (sizeof(F) == 4) {
+                if (4 * g < SLOT) *reinterpret_cast<acc_t*>(part + (int64_t)l * SLOT + 4 * g) = sum;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = Mfma16<F>::row(g, v);
+                    if (c < SLOT) part[(int64_t)l * SLOT + c] = sum[v];
                 }
             }
-        __syncthreads();
-        for (int u = threadIdx.x; u < NOUT; u += T) {
-            const int l = ls + u / SLOT;
-            if (l > L) continue;
-            F sum = 0;
-#pragma unroll
-            for (int ww = 0; ww < NW; ++ww) sum += tb[ww][u];
-            part[(int64_t)ls * SLOT + u] = sum;
         }
         __syncthreads();  // the next super-tile overwrites the buffers
     }
