@@ -32,10 +32,69 @@ __global__ void k_lu_clear(int* __restrict__ flag) {
     if (threadIdx.x == 0) *flag = 0;
 }
 
-// The diagonal block: thread (ti, tc) keeps elements (ti + 16 u, tc + 16 v) in registers;
-// step j publishes the pivot, then column j (divided by the pivot) and row j through LDS.
+// The diagonal block: thread (ti, tc) keeps elements (ti + 16 u, tc + 16 v) in registers.
+// One barrier per step: step j reads the pivot, the raw column j and row j from buffer j & 1
+// (published at the end of step j - 1), every thread forms l = a / pivot for its rows (the
+// column's owner keeps it as L: the same quotient), updates, and the owners of step j + 1's
+// pivot, column and row publish them into the other buffer.
+// Step s's pivot, column (below the pivot) and row sit in register column / row V = s / 16 of
+// the threads with tc = s % 16 / ti = s % 16; V is a template argument so that t stays in
+// registers (a run-time index would move it to private memory).
+template <int V>
+__device__ __forceinline__ void lu_publish_v(const double (&t)[4][4], int s, int b, int ti, int tc, double* pv,
+                                             double* col, double* row) {
+    const int rs = s & 15;
+    if (tc == rs) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = ti + 16 * u;
+            if (i == s) *pv = t[u][V];
+            else if (i > s && i < b) col[i] = t[u][V];
+        }
+    }
+    if (ti == rs) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) row[tc + 16 * v] = t[V][v];
+    }
+}
+
+// Steps j = 16 JB .. 16 JB + 15 (JB a template argument, so every register index is a
+// constant); returns at j = b (uniform over the workgroup).
+template <int JB>
+__device__ __forceinline__ void lu_steps(double (&t)[4][4], bool& bad, int b, int ti, int tc, double (*colj)[kNB],
+                                         double (*rowj)[kNB], double* pv) {
+    for (int r = 0; r < 16; ++r) {
+        const int j = 16 * JB + r;
+        if (j >= b) return;
+        const int q = j & 1;
+        const double piv = pv[q];
+        if (piv == 0.0) bad = true;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = ti + 16 * u;
+            if (i <= j || i >= b) continue;
+            const double lij = colj[q][i] / piv;  // column j: l = a / pivot below the diagonal
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = tc + 16 * v;
+                if (c == j) {
+                    t[u][v] = lij;
+                    if (fabs(lij) > 1.0) bad = true;
+                } else if (c > j && c < b) {
+                    t[u][v] = fma(-lij, rowj[q][c], t[u][v]);
+                }
+            }
+        }
+        if (j + 1 < b) {
+            if (r < 15) lu_publish_v<JB>(t, j + 1, b, ti, tc, &pv[q ^ 1], colj[q ^ 1], rowj[q ^ 1]);
+            else if constexpr (JB < 3) lu_publish_v<JB + 1>(t, j + 1, b, ti, tc, &pv[q ^ 1], colj[q ^ 1], rowj[q ^ 1]);
+        }
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void k_lu_diag(double* __restrict__ A, int n, int k0, int* __restrict__ flag) {
-    __shared__ double colj[kNB], rowj[kNB], pv;
+    __shared__ double colj[2][kNB], rowj[2][kNB], pv[2];
     const int b = min(kNB, n - k0);
     const int tid = threadIdx.x;
     const int ti = tid >> 4, tc = tid & 15;
@@ -48,50 +107,12 @@ __global__ __launch_bounds__(256) void k_lu_diag(double* __restrict__ A, int n, 
             t[u][v] = (i < b && c < b) ? A[(int64_t)(k0 + i) * n + k0 + c] : 0.0;
         }
     bool bad = false;
-    for (int j = 0; j < b; ++j) {
-        const int uj = j >> 4, rj = j & 15;
-        if (ti == rj && tc == rj) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (u == uj) pv = t[u][u];
-        }
-        __syncthreads();
-        const double piv = pv;
-        if (piv == 0.0) bad = true;
-        if (tc == rj) {  // column j: l = a / pivot below the diagonal
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int i = ti + 16 * u;
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    if (v == uj && i > j && i < b) {
-                        const double l = t[u][v] / piv;
-                        t[u][v] = l;
-                        colj[i] = l;
-                        if (fabs(l) > 1.0) bad = true;
-                    }
-            }
-        }
-        if (ti == rj) {  // row j right of the diagonal
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (u == uj)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) rowj[tc + 16 * v] = t[u][v];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = ti + 16 * u;
-            if (i <= j || i >= b) continue;
-            const double lij = colj[i];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = tc + 16 * v;
-                if (c > j && c < b) t[u][v] = fma(-lij, rowj[c], t[u][v]);
-            }
-        }
-    }
+    lu_publish_v<0>(t, 0, b, ti, tc, &pv[0], colj[0], rowj[0]);
+    __syncthreads();
+    lu_steps<0>(t, bad, b, ti, tc, colj, rowj, pv);
+    lu_steps<1>(t, bad, b, ti, tc, colj, rowj, pv);
+    lu_steps<2>(t, bad, b, ti, tc, colj, rowj, pv);
+    lu_steps<3>(t, bad, b, ti, tc, colj, rowj, pv);
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
